@@ -1,0 +1,169 @@
+"""ctypes front end of the C oracle -- TEST INFRASTRUCTURE ONLY.
+
+Imported only by tests/, __graft_entry__.smoke() and bench.py's CPU-baseline
+leg, always as the checker.  See pnfam_oracle.h for which reference lines
+each C function restates.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "_build", "libpnfam_oracle.so")
+
+# pnfam_broyden.f90:20 initialises the mixing factor with a default-real literal
+# (0.7 in single precision, promoted to double).
+ALPHA_MIX = float(np.float32(0.7))
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+class _Problem(C.Structure):
+    _fields_ = [("nb", C.c_int), ("dqp", C.c_int), ("nghl", C.c_int),
+                ("db", _ip), ("isstart", _ip), ("num_spin_up", _ip), ("ns", _ip),
+                ("wf", _dp), ("wfdr", _dp), ("wfdz", _dp), ("wfdp", _dp), ("wfd2_all", _dp),
+                ("wdcori", _dp), ("Ep", _dp), ("En", _dp),
+                ("Up", _dp), ("Vp", _dp), ("Un", _dp), ("Vn", _dp),
+                ("f_ir2c", _ip), ("f_elem", _dp),
+                ("crho", _dp), ("cs", _dp), ("cpair", _dp), ("cspair", _dp), ("cscal", _dp),
+                ("bminus", C.c_int)]
+
+
+class _Opts(C.Structure):
+    _fields_ = [("max_iter", C.c_int), ("eps", C.c_double), ("broyden_history", C.c_int),
+                ("quench", C.c_double), ("alpha_mix", C.c_double)]
+
+
+class _BM(C.Structure):
+    _fields_ = [("ir2c", _ip), ("ic2r", _ip), ("ir2m", _ip), ("ic2m", _ip), ("elem", _dp), ("n", C.c_long)]
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", HERE])
+
+
+def load():
+    if not os.path.exists(LIB):
+        build()
+    lib = C.CDLL(LIB)
+    lib.orc_ifam.restype = C.c_int
+    lib.orc_ifam.argtypes = [C.POINTER(_Problem), C.POINTER(_Opts), C.c_double, C.c_double, _dp, _dp, _dp,
+                             C.c_char_p, _ip, _dp]
+    lib.orc_qp_to_sp.argtypes = [C.POINTER(_Problem), _dp, _dp, _dp, _dp, _dp, _ip]
+    lib.orc_sp_to_qp.argtypes = [C.POINTER(_Problem), _dp, _ip, _dp, _dp, _dp, _dp]
+    lib.orc_fqp.argtypes = [C.POINTER(_Problem), _dp, _dp, _ip, _ip]
+    lib.orc_calc_hamiltonian.argtypes = [C.POINTER(_Problem), C.POINTER(C.POINTER(_BM)),
+                                         C.POINTER(C.POINTER(_BM))]
+    lib.orc_dgemm.argtypes = [C.c_char, C.c_char, C.c_int, C.c_int, C.c_int, C.c_double, _dp, C.c_int, _dp,
+                              C.c_int, C.c_double, _dp, C.c_int]
+    return lib
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    return a.ctypes.data_as(_ip)
+
+
+class Oracle:
+    """Holds the C view of one Problem (keeps the numpy buffers alive)."""
+
+    def __init__(self, prob, lib=None):
+        self.lib = lib or load()
+        b, c = prob.basis, prob.couplings
+        keep = {}
+
+        def dd(name, a):
+            a = np.ascontiguousarray(a, dtype=np.float64) if a.ndim == 1 else np.asfortranarray(a, dtype=np.float64)
+            keep[name] = a
+            return _d(a)
+
+        def ii(name, a):
+            a = np.ascontiguousarray(a, dtype=np.int32)
+            keep[name] = a
+            return _i(a)
+
+        self.p = _Problem(nb=b.nb, dqp=b.dqp, nghl=b.nghl, db=ii("db", b.db), isstart=ii("is", b.isstart),
+                          num_spin_up=ii("nsu", b.num_spin_up), ns=ii("ns", b.ns),
+                          wf=dd("wf", b.wf), wfdr=dd("wfdr", b.wfdr), wfdz=dd("wfdz", b.wfdz),
+                          wfdp=dd("wfdp", b.wfdp), wfd2_all=dd("wfd2", b.wfd2_all), wdcori=dd("wdc", b.grid.wdcori),
+                          Ep=dd("Ep", b.Ep), En=dd("En", b.En), Up=dd("Up", b.Up), Vp=dd("Vp", b.Vp),
+                          Un=dd("Un", b.Un), Vn=dd("Vn", b.Vn), f_ir2c=ii("fmap", prob.f_struct.ir2c),
+                          f_elem=dd("f", prob.f_elem), crho=dd("crho", c.crho), cs=dd("cs", c.cs),
+                          cpair=dd("cpair", c.cpair), cspair=dd("cspair", c.cspair), cscal=dd("cscal", c.scalars()),
+                          bminus=int(prob.bminus))
+        self.keep = keep
+        self.prob = prob
+        self.nxy = prob.nxy
+
+    def ifam(self, omega, max_iter=300, eps=1e-7, broyden_history=50, quench=None, alpha_mix=ALPHA_MIX):
+        q = self.prob.quench if quench is None else quench
+        o = _Opts(max_iter=max_iter, eps=eps, broyden_history=broyden_history, quench=q, alpha_mix=alpha_mix)
+        rs, is_ = C.c_double(), C.c_double()
+        trace = np.zeros(3 * (max_iter + 1))
+        flags = C.create_string_buffer(max_iter + 2)
+        nt = C.c_int()
+        x = np.zeros(4 * self.nxy)
+        conv = self.lib.orc_ifam(C.byref(self.p), C.byref(o), float(np.real(omega)), float(np.imag(omega)),
+                                 C.byref(rs), C.byref(is_), _d(trace), flags, C.byref(nt), _d(x))
+        n = nt.value
+        return {"strength": complex(rs.value, is_.value), "iterations": conv,
+                "trace": trace[:3 * n].reshape(n, 3), "flags": flags.raw[:n].decode(), "x": x}
+
+    def fqp(self):
+        f20, f02 = np.zeros(self.nxy), np.zeros(self.nxy)
+        xm = np.zeros(self.prob.basis.nb, dtype=np.int32)
+        ym = np.zeros_like(xm)
+        self.lib.orc_fqp(C.byref(self.p), _d(f20), _d(f02), _i(xm), _i(ym))
+        return f20, f02, xm, ym
+
+    def qp_to_sp(self, x_re, x_im, y_re, y_im):
+        nb = self.prob.basis.nb
+        rsp = np.zeros(8 * self.nxy)
+        maps = np.zeros(4 * nb, dtype=np.int32)
+        args = [np.ascontiguousarray(a, dtype=np.float64) for a in (x_re, x_im, y_re, y_im)]
+        self.lib.orc_qp_to_sp(C.byref(self.p), *[_d(a) for a in args], _d(rsp), _i(maps))
+        return rsp.reshape(8, self.nxy), maps.reshape(4, nb)
+
+    def sp_to_qp(self, hsp, maps):
+        hsp = np.ascontiguousarray(hsp, dtype=np.float64).ravel()
+        maps = np.ascontiguousarray(maps, dtype=np.int32).ravel()
+        out = [np.zeros(self.nxy) for _ in range(4)]
+        self.lib.orc_sp_to_qp(C.byref(self.p), _d(hsp), _i(maps), *[_d(a) for a in out])
+        return out
+
+    def calc_hamiltonian(self, rsp, maps_in, maps_out):
+        """rsp: (8, nxy) re/im rho_pn, kappa+, rho_np, kappa-; maps_in/out: (8, nb) ir2c."""
+        nb = self.prob.basis.nb
+        db = self.prob.basis.db
+        bms, keep = [], []
+
+        def mk(elem, ir2c):
+            from pynfam_amd.blocks import BlockStruct
+            st = BlockStruct.from_map(db, ir2c)
+            ir2c32 = np.ascontiguousarray(st.ir2c, dtype=np.int32)
+            ic2r = np.ascontiguousarray(st.ic2r, dtype=np.int32)
+            ir2m = np.ascontiguousarray(st.ir2m, dtype=np.int32)
+            ic2m = np.ascontiguousarray(st.ic2m, dtype=np.int32)
+            e = np.ascontiguousarray(elem, dtype=np.float64).copy()
+            keep.extend([ir2c32, ic2r, ir2m, ic2m, e])
+            return _BM(ir2c=_i(ir2c32), ic2r=_i(ic2r), ir2m=_i(ir2m), ic2m=_i(ic2m), elem=_d(e), n=e.shape[0]), e
+
+        ins, outs, out_arrays = [], [], []
+        for k in range(8):
+            bm, _ = mk(rsp[k], maps_in[k])
+            ins.append(bm)
+            bm, e = mk(np.zeros(self.nxy), maps_out[k])
+            outs.append(bm)
+            out_arrays.append(e)
+        PIn = (C.POINTER(_BM) * 8)(*[C.pointer(x) for x in ins])
+        POut = (C.POINTER(_BM) * 8)(*[C.pointer(x) for x in outs])
+        self.lib.orc_calc_hamiltonian(C.byref(self.p), PIn, POut)
+        return np.array(out_arrays)

@@ -1,0 +1,997 @@
+/*
+ * CPU oracle for the pnFAM iteration -- TEST INFRASTRUCTURE ONLY (see header).
+ * FP64 restatement of the reference algorithm; each function names the
+ * reference lines it follows.  Built with gcc -O2 -fopenmp by oracle/Makefile.
+ */
+#include "pnfam_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define PI 3.14159265358979323846
+
+/* ------------------------------------------------------------------------ */
+/* GEMM (column-major), the role BLAS dgemm plays in the reference           */
+/* ------------------------------------------------------------------------ */
+void orc_dgemm(char ta, char tb, int m, int n, int k, double alpha, const double *A, int lda,
+               const double *B, int ldb, double beta, double *C, int ldc) {
+  int tA = (ta == 't' || ta == 'T'), tB = (tb == 't' || tb == 'T');
+  if (m <= 0 || n <= 0) return;
+#pragma omp parallel for schedule(static) if ((long)m * n * k > 200000)
+  for (int j = 0; j < n; ++j) {
+    double *c = C + (long)j * ldc;
+    if (beta == 0.0) {
+      for (int i = 0; i < m; ++i) c[i] = 0.0;
+    } else if (beta != 1.0) {
+      for (int i = 0; i < m; ++i) c[i] *= beta;
+    }
+    if (k <= 0 || alpha == 0.0) continue;
+    if (!tA) {
+      for (int l = 0; l < k; ++l) {
+        double b = tB ? B[j + (long)l * ldb] : B[l + (long)j * ldb];
+        if (b == 0.0) continue;
+        b *= alpha;
+        const double *a = A + (long)l * lda;
+        for (int i = 0; i < m; ++i) c[i] += a[i] * b;
+      }
+    } else {
+      for (int i = 0; i < m; ++i) {
+        const double *a = A + (long)i * lda;
+        double s = 0.0;
+        if (!tB) {
+          const double *b = B + (long)j * ldb;
+          for (int l = 0; l < k; ++l) s += a[l] * b[l];
+        } else {
+          for (int l = 0; l < k; ++l) s += a[l] * B[j + (long)l * ldb];
+        }
+        c[i] += alpha * s;
+      }
+    }
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Block matrices                                                            */
+/* ------------------------------------------------------------------------ */
+static orc_bm *bm_new(int nb, long n) {
+  orc_bm *b = (orc_bm *)calloc(1, sizeof(orc_bm));
+  b->ir2c = (int *)malloc(sizeof(int) * nb);
+  b->ic2r = (int *)malloc(sizeof(int) * nb);
+  b->ir2m = (int *)malloc(sizeof(int) * nb);
+  b->ic2m = (int *)malloc(sizeof(int) * nb);
+  for (int i = 0; i < nb; ++i) b->ir2c[i] = b->ic2r[i] = b->ir2m[i] = b->ic2m[i] = -1;
+  b->elem = (double *)calloc(n > 0 ? n : 1, sizeof(double));
+  b->n = n;
+  return b;
+}
+static void bm_free(orc_bm *b) {
+  if (!b) return;
+  free(b->ir2c); free(b->ic2r); free(b->ir2m); free(b->ic2m); free(b->elem); free(b);
+}
+static void bm_copy_structure(int nb, const orc_bm *a, orc_bm *b) {
+  memcpy(b->ir2c, a->ir2c, sizeof(int) * nb); memcpy(b->ic2r, a->ic2r, sizeof(int) * nb);
+  memcpy(b->ir2m, a->ir2m, sizeof(int) * nb); memcpy(b->ic2m, a->ic2m, sizeof(int) * nb);
+}
+/* Build maps from a row->column map (offsets in row order). */
+static void bm_set_map(int nb, const int *db, orc_bm *b, const int *ir2c) {
+  int off = 0;
+  for (int i = 0; i < nb; ++i) b->ir2c[i] = b->ic2r[i] = b->ir2m[i] = b->ic2m[i] = -1;
+  for (int i = 0; i < nb; ++i) {
+    int c = ir2c[i];
+    if (c < 0) continue;
+    b->ir2c[i] = c; b->ic2r[c] = i; b->ir2m[i] = off; b->ic2m[c] = off;
+    off += db[i] * db[c];
+  }
+}
+
+/* triprod: pnfam_type_blockmatrix.f90:140-211 */
+static void triprod(int nb, const int *db, char ta, const orc_bm *a, char tb, const orc_bm *b,
+                    char tc, const orc_bm *c, double alpha, double beta, orc_bm *abc) {
+  double *aux = (double *)calloc(abc->n > 0 ? abc->n : 1, sizeof(double));
+  for (int i = 0; i < nb; ++i) abc->ir2c[i] = abc->ic2r[i] = abc->ir2m[i] = abc->ic2m[i] = -1;
+  long ipt = 0;
+  for (int i = 0; i < nb; ++i) {
+    int k, l, j, ipa, ipb, ipc;
+    if (ta == 'n') { k = a->ir2c[i]; ipa = a->ir2m[i]; } else { k = a->ic2r[i]; ipa = a->ic2m[i]; }
+    if (k < 0) continue;
+    if (tb == 'n') { l = b->ir2c[k]; ipb = b->ir2m[k]; } else { l = b->ic2r[k]; ipb = b->ic2m[k]; }
+    if (l < 0) continue;
+    if (tc == 'n') { j = c->ir2c[l]; ipc = c->ir2m[l]; } else { j = c->ic2r[l]; ipc = c->ic2m[l]; }
+    if (j < 0) continue;
+    abc->ir2c[i] = j; abc->ic2r[j] = i; abc->ir2m[i] = (int)ipt; abc->ic2m[j] = (int)ipt;
+    int nda = db[i], ndc = db[j];
+    int ldb = (tb == 'n') ? nda : ndc;
+    orc_dgemm(ta, tb, nda, ndc, nda, alpha, a->elem + ipa, nda, b->elem + ipb, ldb, 0.0, aux + ipt, nda);
+    orc_dgemm('n', tc, nda, ndc, ndc, 1.0, aux + ipt, nda, c->elem + ipc, ndc, beta, abc->elem + ipt, nda);
+    ipt += (long)nda * ndc;
+  }
+  free(aux);
+}
+
+/* 2x2 quadrant matrix with per-quadrant transpose/sign flags (pnfam_type_bbm.f90:30-35) */
+typedef struct {
+  orc_bm *m[4]; /* 11 12 21 22 */
+  char t[4];
+  double s[4];
+  int imag;
+} bbm;
+
+static bbm bbm_init(void) {
+  bbm x; memset(&x, 0, sizeof(x));
+  for (int q = 0; q < 4; ++q) { x.t[q] = 'n'; x.s[q] = 1.0; }
+  return x;
+}
+/* transpose_bbm: pnfam_type_bbm.f90:145-156 */
+static bbm bbm_transpose(bbm a) {
+  orc_bm *tm = a.m[1]; a.m[1] = a.m[2]; a.m[2] = tm;
+  double ts = a.s[1]; a.s[1] = a.s[2]; a.s[2] = ts;
+  for (int q = 0; q < 4; ++q) a.t[q] = (a.t[q] == 'n') ? 't' : 'n';
+  return a;
+}
+
+/* triprod_bbm_quad: pnfam_type_bbm.f90:428-552 */
+static void triprod_bbm_quad(int nb, const int *db, char ta, bbm a, char tb, bbm b, char tc, bbm c,
+                             int quadrant, double s, char tabc, orc_bm *out) {
+  if (ta == 't') a = bbm_transpose(a);
+  if (tb == 't') b = bbm_transpose(b);
+  if (tc == 't') c = bbm_transpose(c);
+  if (tabc == 't') {
+    bbm aux = a; a = c; c = aux;
+    a = bbm_transpose(a); b = bbm_transpose(b); c = bbm_transpose(c);
+    if (quadrant == 12) quadrant = 21; else if (quadrant == 21) quadrant = 12;
+  }
+  int im = a.imag + b.imag + c.imag;
+  if (im >= 2) s = -s;
+  memset(out->elem, 0, sizeof(double) * out->n);
+  /* term list: (qa, qb, qc) per output quadrant; index 0=11 1=12 2=21 3=22 */
+  static const int T11[4][3] = {{0, 0, 0}, {1, 2, 0}, {0, 1, 2}, {1, 3, 2}};
+  static const int T12[4][3] = {{0, 0, 1}, {1, 2, 1}, {0, 1, 3}, {1, 3, 3}};
+  static const int T21[4][3] = {{2, 0, 0}, {3, 2, 0}, {2, 1, 2}, {3, 3, 2}};
+  static const int T22[4][3] = {{2, 0, 1}, {3, 2, 1}, {2, 1, 3}, {3, 3, 3}};
+  const int(*T)[3] = quadrant == 11 ? T11 : quadrant == 12 ? T12 : quadrant == 21 ? T21 : T22;
+  for (int t = 0; t < 4; ++t) {
+    int qa = T[t][0], qb = T[t][1], qc = T[t][2];
+    if (a.m[qa] && b.m[qb] && c.m[qc])
+      triprod(nb, db, a.t[qa], a.m[qa], b.t[qb], b.m[qb], c.t[qc], c.m[qc],
+              s * a.s[qa] * b.s[qb] * c.s[qc], 1.0, out);
+  }
+}
+
+/* triprod_bbm: pnfam_type_bbm.f90:557-576 */
+static void triprod_bbm(int nb, const int *db, char ta, bbm a, char tb, bbm b, char tc, bbm c, bbm *abc) {
+  static const int qn[4] = {11, 12, 21, 22};
+  for (int q = 0; q < 4; ++q)
+    if (abc->m[q]) triprod_bbm_quad(nb, db, ta, a, tb, b, tc, c, qn[q], abc->s[q], abc->t[q], abc->m[q]);
+}
+
+/* contract_bbm: pnfam_type_bbm.f90:393-421 (stored signs ignored) */
+static double contract_bbm(const bbm *a, const bbm *b) {
+  double s = 0.0;
+  for (int q = 0; q < 4; ++q)
+    if (a->m[q] && b->m[q]) {
+      double t = 0.0;
+      for (long i = 0; i < a->m[q]->n; ++i) t += a->m[q]->elem[i] * b->m[q]->elem[i];
+      s += t;
+    }
+  if (a->imag + b->imag == 2) s = -s;
+  return s;
+}
+
+/* ------------------------------------------------------------------------ */
+/* calc_hamiltonian (BLAS variant): pnfam_hamiltonian_blas.f90               */
+/* ------------------------------------------------------------------------ */
+enum { D_RHO, D_TAU, D_TJRR, D_TJPR, D_TJZR, D_TJRP, D_TJPP, D_TJZP, D_TJRZ, D_TJPZ, D_TJZZ,
+       D_SR, D_SP, D_SZ, D_TR, D_TP, D_TZ, D_JR, D_JP, D_JZ, D_FR, D_FP, D_FZ, D_GS,
+       D_RB, D_SBR, D_SBP, D_SBZ, ND_ };
+
+/* density: pnfam_hamiltonian_blas.f90:124-711.  ds[2*d + (0 re | 1 im)][nghl] */
+static void density(const orc_problem *p, const orc_bm *rerho, const orc_bm *imrho, const orc_bm *rek,
+                    const orc_bm *imk, double *ds) {
+  const int G = p->nghl, N = p->dqp, nb = p->nb;
+  const long GN = (long)G * N;
+  /* wfa[x][spin 0:+1, 1:-1][re/im] (nghl x dqp), x: 0 wf, 1 dr, 2 dp, 3 dz */
+  double *W = (double *)calloc((size_t)4 * 2 * 2 * GN, sizeof(double));
+#define WFA(x, sp, ri) (W + ((((x) * 2 + (sp)) * 2 + (ri)) * GN))
+  const double *tab[4] = {p->wf, p->wfdr, p->wfdp, p->wfdz};
+  memset(ds, 0, sizeof(double) * 2 * ND_ * G);
+  for (int ix = 0; ix < nb; ++ix) {
+    int iy = rerho->ir2c[ix];
+    if (iy < 0) continue;
+    int ia = p->isstart[ix], ib = p->isstart[iy], nsu = p->num_spin_up[ix], dx = p->db[ix], dy = p->db[iy];
+    for (int x = 0; x < 4; ++x) {
+      if (nsu > 0) {
+        orc_dgemm('n', 'n', G, dy, nsu, 1.0, tab[x] + (long)ia * G, G, rerho->elem + rerho->ir2m[ix], dx, 0.0,
+                  WFA(x, 0, 0) + (long)ib * G, G);
+        orc_dgemm('n', 'n', G, dy, nsu, 1.0, tab[x] + (long)ia * G, G, imrho->elem + imrho->ir2m[ix], dx, 0.0,
+                  WFA(x, 0, 1) + (long)ib * G, G);
+      }
+      if (dx - nsu > 0) {
+        orc_dgemm('n', 'n', G, dy, dx - nsu, 1.0, tab[x] + (long)(ia + nsu) * G, G,
+                  rerho->elem + rerho->ir2m[ix] + nsu, dx, 0.0, WFA(x, 1, 0) + (long)ib * G, G);
+        orc_dgemm('n', 'n', G, dy, dx - nsu, 1.0, tab[x] + (long)(ia + nsu) * G, G,
+                  imrho->elem + imrho->ir2m[ix] + nsu, dx, 0.0, WFA(x, 1, 1) + (long)ib * G, G);
+      }
+    }
+  }
+#define DS(d, ri) (ds + ((long)(2 * (d) + (ri)) * G))
+  for (int b = 0; b < N; ++b) {
+    const int s = p->ns[b];
+    const int sd = s > 0 ? 0 : 1; /* spin index of a == spin of b */
+    const int so = 1 - sd;       /* opposite spin */
+    const double *yb[4] = {p->wf + (long)b * G, p->wfdr + (long)b * G, p->wfdp + (long)b * G,
+                           p->wfdz + (long)b * G};
+    for (int r = 0; r < G; ++r) {
+      double Pr[4][4], Pi[4][4], Qr[4][4], Qi[4][4];
+      for (int x = 0; x < 4; ++x) {
+        double ar = WFA(x, sd, 0)[(long)b * G + r], ai = WFA(x, sd, 1)[(long)b * G + r];
+        double br = WFA(x, so, 0)[(long)b * G + r], bi = WFA(x, so, 1)[(long)b * G + r];
+        for (int y = 0; y < 4; ++y) {
+          double v = yb[y][r];
+          Pr[x][y] = ar * v; Pi[x][y] = ai * v; Qr[x][y] = br * v; Qi[x][y] = bi * v;
+        }
+      }
+      /* diagonal in spin (:263-362) */
+      double auxr, auxi;
+      DS(D_RHO, 0)[r] += Pr[0][0]; DS(D_RHO, 1)[r] += Pi[0][0];
+      DS(D_SZ, 0)[r] += s * Pr[0][0]; DS(D_SZ, 1)[r] += s * Pi[0][0];
+      auxr = (Pr[0][2] + Pr[2][0]) / 2; auxi = (Pi[0][2] + Pi[2][0]) / 2;
+      DS(D_JP, 0)[r] += auxr; DS(D_JP, 1)[r] += auxi;
+      DS(D_TJPZ, 0)[r] += s * auxr; DS(D_TJPZ, 1)[r] += s * auxi;
+      auxr = Pr[1][1] + Pr[2][2] + Pr[3][3]; auxi = Pi[1][1] + Pi[2][2] + Pi[3][3];
+      DS(D_TAU, 0)[r] += auxr; DS(D_TAU, 1)[r] += auxi;
+      DS(D_TZ, 0)[r] += s * auxr; DS(D_TZ, 1)[r] += s * auxi;
+      auxr = (Pr[0][1] - Pr[1][0]) / 2; auxi = (Pi[0][1] - Pi[1][0]) / 2;
+      DS(D_JR, 0)[r] -= auxi; DS(D_JR, 1)[r] += auxr;
+      DS(D_TJRZ, 0)[r] -= s * auxi; DS(D_TJRZ, 1)[r] += s * auxr;
+      auxr = (Pr[0][3] - Pr[3][0]) / 2; auxi = (Pi[0][3] - Pi[3][0]) / 2;
+      DS(D_JZ, 0)[r] -= auxi; DS(D_JZ, 1)[r] += auxr;
+      DS(D_TJZZ, 0)[r] -= s * auxi; DS(D_TJZZ, 1)[r] += s * auxr;
+      auxr = Pr[0][3] + Pr[3][0]; auxi = Pi[0][3] + Pi[3][0];
+      DS(D_GS, 0)[r] += s * auxr; DS(D_GS, 1)[r] += s * auxi;
+      auxr = Pr[1][3] + Pr[3][1]; auxi = Pi[1][3] + Pi[3][1];
+      DS(D_FR, 0)[r] += s * auxr / 2; DS(D_FR, 1)[r] += s * auxi / 2;
+      auxr = Pr[2][3] - Pr[3][2]; auxi = Pi[2][3] - Pi[3][2];
+      DS(D_FP, 0)[r] -= s * auxi / 2; DS(D_FP, 1)[r] += s * auxr / 2;
+      DS(D_FZ, 0)[r] += s * Pr[3][3]; DS(D_FZ, 1)[r] += s * Pi[3][3];
+      /* not diagonal in spin (:366-581) */
+      DS(D_SR, 0)[r] += Qr[0][0]; DS(D_SR, 1)[r] += Qi[0][0];
+      double t2r = (Qr[2][0] + Qr[0][2]) / 2, t2i = (Qi[2][0] + Qi[0][2]) / 2;
+      DS(D_TJPR, 0)[r] += t2r; DS(D_TJPR, 1)[r] += t2i;
+      double tkr = Qr[1][1] + Qr[2][2] + Qr[3][3], tki = Qi[1][1] + Qi[2][2] + Qi[3][3];
+      DS(D_TR, 0)[r] += tkr; DS(D_TR, 1)[r] += tki;
+      double jrr = (Qr[0][1] - Qr[1][0]) / 2, jri = (Qi[0][1] - Qi[1][0]) / 2;
+      DS(D_TJRR, 0)[r] -= jri; DS(D_TJRR, 1)[r] += jrr;
+      DS(D_GS, 0)[r] += Qr[0][1] + Qr[1][0]; DS(D_GS, 1)[r] += Qi[0][1] + Qi[1][0];
+      double jzr = (Qr[0][3] - Qr[3][0]) / 2, jzi = (Qi[0][3] - Qi[3][0]) / 2;
+      DS(D_TJZR, 0)[r] -= jzi; DS(D_TJZR, 1)[r] += jzr;
+      double gpr = Qr[0][2] - Qr[2][0], gpi = Qi[0][2] - Qi[2][0];
+      if (s == -1) { /* |a>=|+>, |b>=|-> */
+        DS(D_SP, 0)[r] -= Qi[0][0]; DS(D_SP, 1)[r] += Qr[0][0];
+        DS(D_TJPP, 0)[r] -= t2i; DS(D_TJPP, 1)[r] += t2r;
+        DS(D_GS, 0)[r] += gpr; DS(D_GS, 1)[r] += gpi;
+        DS(D_TP, 0)[r] -= tki; DS(D_TP, 1)[r] += tkr;
+        DS(D_TJRP, 0)[r] -= jrr; DS(D_TJRP, 1)[r] -= jri;
+        DS(D_TJZP, 0)[r] -= jzr; DS(D_TJZP, 1)[r] -= jzi;
+        auxr = Qr[1][1] + 0.5 * Qr[1][2] - 0.5 * Qr[2][1];
+        auxi = Qi[1][1] + 0.5 * Qi[1][2] - 0.5 * Qi[2][1];
+        DS(D_FR, 0)[r] += auxr; DS(D_FR, 1)[r] += auxi;
+        auxr = Qr[2][2] + 0.5 * (Qr[2][1] - Qr[1][2]);
+        auxi = Qi[2][2] + 0.5 * (Qi[2][1] - Qi[1][2]);
+        DS(D_FP, 0)[r] -= auxi; DS(D_FP, 1)[r] += auxr;
+        auxr = Qr[3][1] + Qr[3][2] + Qr[1][3] - Qr[2][3];
+        auxi = Qi[3][1] + Qi[3][2] + Qi[1][3] - Qi[2][3];
+        DS(D_FZ, 0)[r] += auxr / 2; DS(D_FZ, 1)[r] += auxi / 2;
+      } else { /* |a>=|->, |b>=|+> */
+        DS(D_SP, 0)[r] += Qi[0][0]; DS(D_SP, 1)[r] -= Qr[0][0];
+        DS(D_TJPP, 0)[r] += t2i; DS(D_TJPP, 1)[r] -= t2r;
+        DS(D_GS, 0)[r] -= gpr; DS(D_GS, 1)[r] -= gpi;
+        DS(D_TP, 0)[r] += tki; DS(D_TP, 1)[r] -= tkr;
+        DS(D_TJRP, 0)[r] += jrr; DS(D_TJRP, 1)[r] += jri;
+        DS(D_TJZP, 0)[r] += jzr; DS(D_TJZP, 1)[r] += jzi;
+        auxr = Qr[1][1] - 0.5 * Qr[1][2] + 0.5 * Qr[2][1];
+        auxi = Qi[1][1] - 0.5 * Qi[1][2] + 0.5 * Qi[2][1];
+        DS(D_FR, 0)[r] += auxr; DS(D_FR, 1)[r] += auxi;
+        auxr = -Qr[2][2] + 0.5 * (Qr[2][1] - Qr[1][2]);
+        auxi = -Qi[2][2] + 0.5 * (Qi[2][1] - Qi[1][2]);
+        DS(D_FP, 0)[r] -= auxi; DS(D_FP, 1)[r] += auxr;
+        auxr = Qr[3][1] - Qr[3][2] + Qr[1][3] + Qr[2][3];
+        auxi = Qi[3][1] - Qi[3][2] + Qi[1][3] + Qi[2][3];
+        DS(D_FZ, 0)[r] += auxr / 2; DS(D_FZ, 1)[r] += auxi / 2;
+      }
+    }
+  }
+  /* pairing densities (:596-675) */
+  memset(WFA(0, 0, 0), 0, sizeof(double) * 4 * GN); /* wfa(spin +/-, re/im) for x = 0 */
+  for (int ix = 0; ix < nb; ++ix) {
+    int iy = rek->ir2c[ix];
+    if (iy < 0) continue;
+    int ia = p->isstart[ix], ib = p->isstart[iy], nsu = p->num_spin_up[ix], dx = p->db[ix], dy = p->db[iy];
+    if (nsu > 0) {
+      orc_dgemm('n', 'n', G, dy, nsu, 1.0, p->wf + (long)ia * G, G, rek->elem + rek->ir2m[ix], dx, 0.0,
+                WFA(0, 0, 0) + (long)ib * G, G);
+      orc_dgemm('n', 'n', G, dy, nsu, 1.0, p->wf + (long)ia * G, G, imk->elem + imk->ir2m[ix], dx, 0.0,
+                WFA(0, 0, 1) + (long)ib * G, G);
+    }
+    if (dx - nsu > 0) {
+      orc_dgemm('n', 'n', G, dy, dx - nsu, 1.0, p->wf + (long)(ia + nsu) * G, G, rek->elem + rek->ir2m[ix] + nsu,
+                dx, 0.0, WFA(0, 1, 0) + (long)ib * G, G);
+      orc_dgemm('n', 'n', G, dy, dx - nsu, 1.0, p->wf + (long)(ia + nsu) * G, G, imk->elem + imk->ir2m[ix] + nsu,
+                dx, 0.0, WFA(0, 1, 1) + (long)ib * G, G);
+    }
+  }
+  for (int b = 0; b < N; ++b) {
+    const int s = p->ns[b];
+    const int sd = s > 0 ? 0 : 1, so = 1 - sd;
+    const double *wb = p->wf + (long)b * G;
+    for (int r = 0; r < G; ++r) {
+      double pr = 2 * WFA(0, so, 0)[(long)b * G + r] * wb[r], pi = 2 * WFA(0, so, 1)[(long)b * G + r] * wb[r];
+      DS(D_RB, 0)[r] -= s * pr; DS(D_RB, 1)[r] -= s * pi;
+      DS(D_SBZ, 0)[r] += pr; DS(D_SBZ, 1)[r] += pi;
+      pr = 2 * WFA(0, sd, 0)[(long)b * G + r] * wb[r]; pi = 2 * WFA(0, sd, 1)[(long)b * G + r] * wb[r];
+      if (s == -1) { DS(D_SBR, 0)[r] += pr; DS(D_SBR, 1)[r] += pi; }
+      else { DS(D_SBR, 0)[r] -= pr; DS(D_SBR, 1)[r] -= pi; }
+      DS(D_SBP, 0)[r] += pi; DS(D_SBP, 1)[r] -= pr;
+    }
+  }
+  for (int d = 0; d < ND_; ++d)
+    for (int ri = 0; ri < 2; ++ri)
+      for (int r = 0; r < G; ++r) DS(d, ri)[r] *= p->wdcori[r];
+#undef WFA
+  free(W);
+}
+
+/* complex field on the grid */
+typedef struct { double *re, *im; int on; } fld;
+
+/* meanfield: pnfam_hamiltonian_blas.f90:717-1169 */
+static void meanfield(const orc_problem *p, const double *ds, orc_bm *reh, orc_bm *imh) {
+  const int G = p->nghl, N = p->dqp, nb = p->nb;
+  const double *cs = p->cs, *crho = p->crho;
+  const double cdrho = p->cscal[0], ctau = p->cscal[1], ctj0 = p->cscal[2], ctj1 = p->cscal[3],
+               ctj2 = p->cscal[4], crdj = p->cscal[5], cds = p->cscal[6], ct = p->cscal[7], cj = p->cscal[8],
+               cgs = p->cscal[9], cf = p->cscal[10], csdj = p->cscal[11];
+  memset(reh->elem, 0, sizeof(double) * reh->n);
+  memset(imh->elem, 0, sizeof(double) * imh->n);
+  /* mf(ia 0:4, ib 0:4, sa, sb); spin index 0:+1 1:-1 */
+  static fld mf[5][5][2][2];
+  double *store = (double *)calloc((size_t)5 * 5 * 2 * 2 * 2 * G, sizeof(double));
+  for (int a = 0; a < 5; ++a)
+    for (int b = 0; b < 5; ++b)
+      for (int x = 0; x < 2; ++x)
+        for (int y = 0; y < 2; ++y) {
+          long o = ((((long)a * 5 + b) * 2 + x) * 2 + y) * 2 * G;
+          mf[a][b][x][y].re = store + o; mf[a][b][x][y].im = store + o + G;
+          mf[a][b][x][y].on = (a < 4 && b < 4) || (a == 0 && b == 4) || (a == 4 && b == 0);
+        }
+  double *tmp = (double *)calloc((size_t)12 * G, sizeof(double));
+  fld aux = {tmp, tmp + G, 1};
+  fld c0 = {tmp + 2 * G, tmp + 3 * G, 1};   /* ctj0*(tjrr+tjpp+tjzz) */
+  fld c1a = {tmp + 4 * G, tmp + 5 * G, 1};  /* ctj1*(tjzr-tjrz) */
+  fld c1b = {tmp + 6 * G, tmp + 7 * G, 1};  /* ctj1*(tjpz-tjzp) */
+  fld c2a = {tmp + 8 * G, tmp + 9 * G, 1};  /* ctj2*(tjrz+tjzr) */
+  fld c2b = {tmp + 10 * G, tmp + 11 * G, 1}; /* ctj2*(tjpz+tjzp) */
+#define R(d) (ds + (long)(2 * (d)) * G)
+#define I(d) (ds + (long)(2 * (d) + 1) * G)
+#define P 0
+#define M 1
+#define MF(a, b, x, y) mf[a][b][x][y]
+#define FOR for (int r = 0; r < G; ++r)
+#define SET(f, exre, exim) FOR { (f).re[r] = (exre); (f).im[r] = (exim); }
+#define CPY(dst, src) FOR { (dst).re[r] = (src).re[r]; (dst).im[r] = (src).im[r]; }
+#define ADD(dst, sym, src) FOR { (dst).re[r] += (sym) * (src).re[r]; (dst).im[r] += (sym) * (src).im[r]; }
+#define NEG(dst, src) FOR { (dst).re[r] = -(src).re[r]; (dst).im[r] = -(src).im[r]; }
+  SET(c0, ctj0 * (R(D_TJRR)[r] + R(D_TJPP)[r] + R(D_TJZZ)[r]), ctj0 * (I(D_TJRR)[r] + I(D_TJPP)[r] + I(D_TJZZ)[r]));
+  SET(c1a, ctj1 * (R(D_TJZR)[r] - R(D_TJRZ)[r]), ctj1 * (I(D_TJZR)[r] - I(D_TJRZ)[r]));
+  SET(c1b, ctj1 * (R(D_TJPZ)[r] - R(D_TJZP)[r]), ctj1 * (I(D_TJPZ)[r] - I(D_TJZP)[r]));
+  SET(c2a, ctj2 * (R(D_TJRZ)[r] + R(D_TJZR)[r]), ctj2 * (I(D_TJRZ)[r] + I(D_TJZR)[r]));
+  SET(c2b, ctj2 * (R(D_TJPZ)[r] + R(D_TJZP)[r]), ctj2 * (I(D_TJPZ)[r] + I(D_TJZP)[r]));
+
+  /* wf_a, wf_b, same spin */
+  SET(MF(0, 0, P, P), 2 * crho[r] * R(D_RHO)[r] + ctau * R(D_TAU)[r], 2 * crho[r] * I(D_RHO)[r] + ctau * I(D_TAU)[r]);
+  CPY(MF(0, 0, M, M), MF(0, 0, P, P));
+  SET(aux, 2 * cs[r] * R(D_SZ)[r] + ct * R(D_TZ)[r] + cf * R(D_FZ)[r],
+      2 * cs[r] * I(D_SZ)[r] + ct * I(D_TZ)[r] + cf * I(D_FZ)[r]);
+  ADD(MF(0, 0, P, P), 1, aux); ADD(MF(0, 0, M, M), -1, aux);
+  /* (0,1) and (1,0) same spin */
+  SET(MF(0, 1, P, P), -crdj * (R(D_TJPZ)[r] - R(D_TJZP)[r]), -crdj * (I(D_TJPZ)[r] - I(D_TJZP)[r]));
+  CPY(MF(0, 1, M, M), MF(0, 1, P, P));
+  SET(aux, -csdj * R(D_JP)[r], -csdj * I(D_JP)[r]);
+  ADD(MF(0, 1, P, P), 1, aux); ADD(MF(0, 1, M, M), -1, aux);
+  CPY(MF(1, 0, P, P), MF(0, 1, P, P)); CPY(MF(1, 0, M, M), MF(0, 1, M, M));
+  SET(aux, -c1a.im[r] + 0.5 * c2a.im[r], c1a.re[r] - 0.5 * c2a.re[r]);
+  ADD(MF(0, 1, P, P), 1, aux); ADD(MF(0, 1, M, M), -1, aux); ADD(MF(1, 0, P, P), -1, aux); ADD(MF(1, 0, M, M), 1, aux);
+  SET(aux, cj * I(D_JR)[r], -cj * R(D_JR)[r]);
+  ADD(MF(0, 1, P, P), 1, aux); ADD(MF(0, 1, M, M), 1, aux); ADD(MF(1, 0, P, P), -1, aux); ADD(MF(1, 0, M, M), -1, aux);
+  /* (0,2) and (2,0) same spin */
+  SET(MF(0, 2, P, P), cj * R(D_JP)[r], cj * I(D_JP)[r]);
+  CPY(MF(0, 2, M, M), MF(0, 2, P, P));
+  SET(aux, c1b.re[r] + 0.5 * c2b.re[r], c1b.im[r] + 0.5 * c2b.im[r]);
+  ADD(MF(0, 2, P, P), 1, aux); ADD(MF(0, 2, M, M), -1, aux);
+  CPY(MF(2, 0, P, P), MF(0, 2, P, P)); CPY(MF(2, 0, M, M), MF(0, 2, M, M));
+  SET(aux, -csdj * I(D_JR)[r], csdj * R(D_JR)[r]);
+  ADD(MF(0, 2, P, P), 1, aux); ADD(MF(0, 2, M, M), -1, aux); ADD(MF(2, 0, P, P), -1, aux); ADD(MF(2, 0, M, M), 1, aux);
+  SET(aux, crdj * (I(D_TJZR)[r] - I(D_TJRZ)[r]), -crdj * (R(D_TJZR)[r] - R(D_TJRZ)[r]));
+  ADD(MF(0, 2, P, P), 1, aux); ADD(MF(0, 2, M, M), 1, aux); ADD(MF(2, 0, P, P), -1, aux); ADD(MF(2, 0, M, M), -1, aux);
+  /* (0,3) and (3,0) same spin */
+  SET(MF(0, 3, P, P), -crdj * (R(D_TJRP)[r] - R(D_TJPR)[r]), -crdj * (I(D_TJRP)[r] - I(D_TJPR)[r]));
+  CPY(MF(0, 3, M, M), MF(0, 3, P, P));
+  SET(aux, 2 * cgs * R(D_GS)[r], 2 * cgs * I(D_GS)[r]);
+  ADD(MF(0, 3, P, P), 1, aux); ADD(MF(0, 3, M, M), -1, aux);
+  CPY(MF(3, 0, P, P), MF(0, 3, P, P)); CPY(MF(3, 0, M, M), MF(0, 3, M, M));
+  SET(aux, c0.im[r] - ctj2 * (I(D_TJRR)[r] + I(D_TJPP)[r] - 2 * I(D_TJZZ)[r]) / 3.0,
+      -c0.re[r] + ctj2 * (R(D_TJRR)[r] + R(D_TJPP)[r] - 2 * R(D_TJZZ)[r]) / 3.0);
+  ADD(MF(0, 3, P, P), 1, aux); ADD(MF(0, 3, M, M), -1, aux); ADD(MF(3, 0, P, P), -1, aux); ADD(MF(3, 0, M, M), 1, aux);
+  SET(aux, cj * I(D_JZ)[r], -cj * R(D_JZ)[r]);
+  ADD(MF(0, 3, P, P), 1, aux); ADD(MF(0, 3, M, M), 1, aux); ADD(MF(3, 0, P, P), -1, aux); ADD(MF(3, 0, M, M), -1, aux);
+  /* (0,4) and (4,0) same spin */
+  SET(MF(0, 4, P, P), 2 * cdrho * R(D_RHO)[r], 2 * cdrho * I(D_RHO)[r]);
+  CPY(MF(0, 4, M, M), MF(0, 4, P, P));
+  SET(aux, 2 * cds * R(D_SZ)[r], 2 * cds * I(D_SZ)[r]);
+  ADD(MF(0, 4, P, P), 1, aux); ADD(MF(0, 4, M, M), -1, aux);
+  CPY(MF(4, 0, P, P), MF(0, 4, P, P)); CPY(MF(4, 0, M, M), MF(0, 4, M, M));
+  /* (1,2) and (2,1) same spin */
+  SET(MF(1, 2, P, P), csdj * R(D_SZ)[r], csdj * I(D_SZ)[r]);
+  CPY(MF(1, 2, M, M), MF(1, 2, P, P));
+  SET(aux, crdj * R(D_RHO)[r], crdj * I(D_RHO)[r]);
+  ADD(MF(1, 2, P, P), 1, aux); ADD(MF(1, 2, M, M), -1, aux);
+  CPY(MF(2, 1, P, P), MF(1, 2, P, P)); CPY(MF(2, 1, M, M), MF(1, 2, M, M));
+  /* (1,3) and (3,1) same spin */
+  SET(MF(1, 3, P, P), 0.5 * cf * R(D_SR)[r], 0.5 * cf * I(D_SR)[r]);
+  NEG(MF(1, 3, M, M), MF(1, 3, P, P));
+  CPY(MF(3, 1, P, P), MF(1, 3, P, P)); CPY(MF(3, 1, M, M), MF(1, 3, M, M));
+  SET(aux, -csdj * I(D_SP)[r], csdj * R(D_SP)[r]);
+  ADD(MF(1, 3, P, P), 1, aux); ADD(MF(1, 3, M, M), 1, aux); ADD(MF(3, 1, P, P), -1, aux); ADD(MF(3, 1, M, M), -1, aux);
+  /* (2,3) and (3,2) same spin */
+  SET(MF(2, 3, P, P), -csdj * R(D_SR)[r], -csdj * I(D_SR)[r]);
+  CPY(MF(2, 3, M, M), MF(2, 3, P, P));
+  SET(aux, 0.5 * cf * I(D_SP)[r], -0.5 * cf * R(D_SP)[r]);
+  ADD(MF(2, 3, P, P), 1, aux); ADD(MF(2, 3, M, M), -1, aux);
+  CPY(MF(3, 2, P, P), MF(2, 3, M, M)); CPY(MF(3, 2, M, M), MF(2, 3, P, P));
+  /* (1,1), (2,2), (3,3) same spin */
+  SET(MF(1, 1, P, P), (4 * cdrho + ctau) * R(D_RHO)[r], (4 * cdrho + ctau) * I(D_RHO)[r]);
+  CPY(MF(1, 1, M, M), MF(1, 1, P, P));
+  SET(aux, (4 * cds + ct) * R(D_SZ)[r], (4 * cds + ct) * I(D_SZ)[r]);
+  ADD(MF(1, 1, P, P), 1, aux); ADD(MF(1, 1, M, M), -1, aux);
+  CPY(MF(2, 2, P, P), MF(1, 1, P, P)); CPY(MF(2, 2, M, M), MF(1, 1, M, M));
+  CPY(MF(3, 3, P, P), MF(1, 1, P, P)); CPY(MF(3, 3, M, M), MF(1, 1, M, M));
+  SET(aux, cf * R(D_SZ)[r], cf * I(D_SZ)[r]);
+  ADD(MF(3, 3, P, P), 1, aux); ADD(MF(3, 3, M, M), -1, aux);
+
+  /* opposite spins: (0,0) */
+  SET(MF(0, 0, P, M), 2 * cs[r] * R(D_SR)[r] + ct * R(D_TR)[r] + cf * R(D_FR)[r],
+      2 * cs[r] * I(D_SR)[r] + ct * I(D_TR)[r] + cf * I(D_FR)[r]);
+  CPY(MF(0, 0, M, P), MF(0, 0, P, M));
+  SET(aux, 2 * cs[r] * I(D_SP)[r] + ct * I(D_TP)[r] + cf * I(D_FP)[r],
+      -2 * cs[r] * R(D_SP)[r] - ct * R(D_TP)[r] - cf * R(D_FP)[r]);
+  ADD(MF(0, 0, P, M), 1, aux); ADD(MF(0, 0, M, P), -1, aux);
+  /* (0,1), (1,0), (0,2), (2,0) opposite */
+  SET(MF(0, 1, P, M), 2 * cgs * R(D_GS)[r], 2 * cgs * I(D_GS)[r]);
+  CPY(MF(0, 1, M, P), MF(0, 1, P, M));
+  SET(aux, csdj * I(D_JZ)[r], -csdj * R(D_JZ)[r]);
+  ADD(MF(0, 1, P, M), 1, aux); ADD(MF(0, 1, M, P), -1, aux);
+  CPY(MF(1, 0, P, M), MF(0, 1, P, M)); CPY(MF(1, 0, M, P), MF(0, 1, M, P));
+  CPY(MF(0, 2, P, M), MF(0, 1, P, M)); CPY(MF(2, 0, M, P), MF(1, 0, M, P));
+  NEG(MF(2, 0, P, M), MF(0, 2, P, M)); NEG(MF(0, 2, M, P), MF(2, 0, M, P));
+  SET(aux, -ctj1 * (R(D_TJRP)[r] - R(D_TJPR)[r]), -ctj1 * (I(D_TJRP)[r] - I(D_TJPR)[r]));
+  ADD(MF(0, 1, P, M), 1, aux); ADD(MF(0, 1, M, P), -1, aux); ADD(MF(1, 0, P, M), -1, aux); ADD(MF(1, 0, M, P), 1, aux);
+  ADD(MF(0, 2, P, M), 1, aux); ADD(MF(0, 2, M, P), 1, aux); ADD(MF(2, 0, P, M), 1, aux); ADD(MF(2, 0, M, P), 1, aux);
+  SET(aux, c0.im[r], -c0.re[r]);
+  ADD(MF(0, 1, P, M), 1, aux); ADD(MF(0, 1, M, P), 1, aux); ADD(MF(1, 0, P, M), -1, aux); ADD(MF(1, 0, M, P), -1, aux);
+  ADD(MF(0, 2, P, M), 1, aux); ADD(MF(0, 2, M, P), -1, aux); ADD(MF(2, 0, P, M), 1, aux); ADD(MF(2, 0, M, P), -1, aux);
+  SET(aux, -0.5 * ctj2 * (R(D_TJRP)[r] + R(D_TJPR)[r]), -0.5 * ctj2 * (I(D_TJRP)[r] + I(D_TJPR)[r]));
+  ADD(MF(0, 1, P, M), 1, aux); ADD(MF(0, 1, M, P), -1, aux); ADD(MF(1, 0, P, M), -1, aux); ADD(MF(1, 0, M, P), 1, aux);
+  ADD(MF(0, 2, P, M), -1, aux); ADD(MF(0, 2, M, P), -1, aux); ADD(MF(2, 0, P, M), -1, aux); ADD(MF(2, 0, M, P), -1, aux);
+  SET(aux, -ctj2 * (-2 * I(D_TJRR)[r] + I(D_TJPP)[r] + I(D_TJZZ)[r]) / 3.0,
+      ctj2 * (-2 * R(D_TJRR)[r] + R(D_TJPP)[r] + R(D_TJZZ)[r]) / 3.0);
+  ADD(MF(0, 1, P, M), 1, aux); ADD(MF(0, 1, M, P), 1, aux); ADD(MF(1, 0, P, M), -1, aux); ADD(MF(1, 0, M, P), -1, aux);
+  SET(aux, -ctj2 * (I(D_TJRR)[r] - 2 * I(D_TJPP)[r] + I(D_TJZZ)[r]) / 3.0,
+      ctj2 * (R(D_TJRR)[r] - 2 * R(D_TJPP)[r] + R(D_TJZZ)[r]) / 3.0);
+  ADD(MF(0, 2, P, M), 1, aux); ADD(MF(0, 2, M, P), -1, aux); ADD(MF(2, 0, P, M), 1, aux); ADD(MF(2, 0, M, P), -1, aux);
+  /* (0,3), (3,0) opposite */
+  SET(MF(0, 3, P, M), csdj * R(D_JP)[r], csdj * I(D_JP)[r]);
+  CPY(MF(0, 3, M, P), MF(0, 3, P, M));
+  SET(aux, -csdj * I(D_JR)[r], csdj * R(D_JR)[r]);
+  ADD(MF(0, 3, P, M), 1, aux); ADD(MF(0, 3, M, P), -1, aux);
+  CPY(MF(3, 0, P, M), MF(0, 3, P, M)); CPY(MF(3, 0, M, P), MF(0, 3, M, P));
+  SET(aux, c1a.im[r] + 0.5 * c2a.im[r], -c1a.re[r] - 0.5 * c2a.re[r]);
+  ADD(MF(0, 3, P, M), 1, aux); ADD(MF(0, 3, M, P), 1, aux); ADD(MF(3, 0, P, M), -1, aux); ADD(MF(3, 0, M, P), -1, aux);
+  SET(aux, c1b.re[r] - 0.5 * c2b.re[r], c1b.im[r] - 0.5 * c2b.im[r]);
+  ADD(MF(0, 3, P, M), 1, aux); ADD(MF(0, 3, M, P), -1, aux); ADD(MF(3, 0, P, M), -1, aux); ADD(MF(3, 0, M, P), 1, aux);
+  /* (0,4), (4,0) opposite */
+  SET(MF(0, 4, P, M), 2 * cds * R(D_SR)[r], 2 * cds * I(D_SR)[r]);
+  CPY(MF(0, 4, M, P), MF(0, 4, P, M));
+  SET(aux, 2 * cds * I(D_SP)[r], -2 * cds * R(D_SP)[r]);
+  ADD(MF(0, 4, P, M), 1, aux); ADD(MF(0, 4, M, P), -1, aux);
+  CPY(MF(4, 0, P, M), MF(0, 4, P, M)); CPY(MF(4, 0, M, P), MF(0, 4, M, P));
+  /* (1,2), (2,1) opposite */
+  SET(MF(1, 2, P, M), -0.5 * cf * I(D_SP)[r], 0.5 * cf * R(D_SP)[r]);
+  CPY(MF(1, 2, M, P), MF(1, 2, P, M));
+  SET(aux, 0.5 * cf * R(D_SR)[r], 0.5 * cf * I(D_SR)[r]);
+  ADD(MF(1, 2, P, M), 1, aux); ADD(MF(1, 2, M, P), -1, aux);
+  NEG(MF(2, 1, P, M), MF(1, 2, P, M)); NEG(MF(2, 1, M, P), MF(1, 2, M, P));
+  /* (1,3), (3,1), (2,3), (3,2) opposite */
+  SET(MF(1, 3, P, M), 0.5 * cf * R(D_SZ)[r], 0.5 * cf * I(D_SZ)[r]);
+  CPY(MF(1, 3, M, P), MF(1, 3, P, M));
+  SET(aux, crdj * R(D_RHO)[r], crdj * I(D_RHO)[r]);
+  ADD(MF(1, 3, P, M), 1, aux); ADD(MF(1, 3, M, P), -1, aux);
+  CPY(MF(3, 1, P, M), MF(1, 3, M, P)); CPY(MF(3, 1, M, P), MF(1, 3, P, M));
+  CPY(MF(3, 2, P, M), MF(3, 1, P, M)); CPY(MF(2, 3, M, P), MF(1, 3, M, P));
+  NEG(MF(2, 3, P, M), MF(1, 3, P, M)); NEG(MF(3, 2, M, P), MF(3, 1, M, P));
+  /* (3,3), (1,1), (2,2) opposite */
+  SET(MF(3, 3, P, M), (ct + 4 * cds) * R(D_SR)[r], (ct + 4 * cds) * I(D_SR)[r]);
+  CPY(MF(3, 3, M, P), MF(3, 3, P, M));
+  SET(aux, (ct + 4 * cds) * I(D_SP)[r], -(ct + 4 * cds) * R(D_SP)[r]);
+  ADD(MF(3, 3, P, M), 1, aux); ADD(MF(3, 3, M, P), -1, aux);
+  CPY(MF(1, 1, P, M), MF(3, 3, P, M)); CPY(MF(1, 1, M, P), MF(3, 3, M, P));
+  SET(aux, cf * R(D_SR)[r], cf * I(D_SR)[r]);
+  ADD(MF(1, 1, P, M), 1, aux); ADD(MF(1, 1, M, P), 1, aux);
+  CPY(MF(2, 2, P, M), MF(3, 3, P, M)); CPY(MF(2, 2, M, P), MF(3, 3, M, P));
+  SET(aux, cf * I(D_SP)[r], -cf * R(D_SP)[r]);
+  ADD(MF(2, 2, P, M), 1, aux); ADD(MF(2, 2, M, P), -1, aux);
+#undef SET
+#undef CPY
+#undef ADD
+#undef NEG
+#undef MF
+#undef FOR
+#undef P
+#undef M
+
+  /* hpsi(iy, sa)(:, b) = sum_iz mf(iy, iz, sa, sb) * psi_iz(:, b)   (:1105-1127) */
+  const long GN = (long)G * N;
+  double *hp = (double *)calloc((size_t)5 * 2 * 2 * GN, sizeof(double));
+#define HP(iy, sa, ri) (hp + ((((iy) * 2 + (sa)) * 2 + (ri)) * GN))
+  const double *psi[5] = {p->wf, p->wfdr, p->wfdp, p->wfdz, p->wfd2_all};
+#pragma omp parallel for schedule(static)
+  for (int b = 0; b < N; ++b) {
+    int sb = p->ns[b] > 0 ? 0 : 1;
+    for (int sa = 0; sa < 2; ++sa)
+      for (int iy = 0; iy < 5; ++iy)
+        for (int iz = 0; iz < 5; ++iz) {
+          const fld *f = &mf[iy][iz][sa][sb];
+          if (!f->on) continue;
+          const double *pz = psi[iz] + (long)b * G;
+          double *hr = HP(iy, sa, 0) + (long)b * G, *hi = HP(iy, sa, 1) + (long)b * G;
+          for (int r = 0; r < G; ++r) { hr[r] += f->re[r] * pz[r]; hi[r] += f->im[r] * pz[r]; }
+        }
+  }
+  /* h_ab = 2 sum_iz psi_iz(:,a)^T hpsi(iz, s_a)(:, b)   (:1131-1167) */
+  for (int ix = 0; ix < nb; ++ix) {
+    int iy = reh->ir2c[ix];
+    if (iy < 0) continue;
+    int ia = p->isstart[ix], ib = p->isstart[iy], nsu = p->num_spin_up[ix], dx = p->db[ix], dy = p->db[iy];
+    for (int iz = 0; iz < 5; ++iz) {
+      if (nsu > 0) {
+        orc_dgemm('t', 'n', nsu, dy, G, 2.0, psi[iz] + (long)ia * G, G, HP(iz, 0, 0) + (long)ib * G, G, 1.0,
+                  reh->elem + reh->ir2m[ix], dx);
+        orc_dgemm('t', 'n', nsu, dy, G, 2.0, psi[iz] + (long)ia * G, G, HP(iz, 0, 1) + (long)ib * G, G, 1.0,
+                  imh->elem + imh->ir2m[ix], dx);
+      }
+      if (dx - nsu > 0) {
+        orc_dgemm('t', 'n', dx - nsu, dy, G, 2.0, psi[iz] + (long)(ia + nsu) * G, G, HP(iz, 1, 0) + (long)ib * G, G,
+                  1.0, reh->elem + reh->ir2m[ix] + nsu, dx);
+        orc_dgemm('t', 'n', dx - nsu, dy, G, 2.0, psi[iz] + (long)(ia + nsu) * G, G, HP(iz, 1, 1) + (long)ib * G, G,
+                  1.0, imh->elem + imh->ir2m[ix] + nsu, dx);
+      }
+    }
+  }
+#undef HP
+  free(hp);
+  free(tmp);
+  free(store);
+}
+
+/* pairingfield: pnfam_hamiltonian_blas.f90:1175-1262 */
+static void pairingfield(const orc_problem *p, const double *ds, orc_bm *red, orc_bm *imd) {
+  const int G = p->nghl, N = p->dqp, nb = p->nb;
+  const double *cpair = p->cpair, *cspair = p->cspair;
+  memset(red->elem, 0, sizeof(double) * red->n);
+  memset(imd->elem, 0, sizeof(double) * imd->n);
+  double *aux = (double *)malloc(sizeof(double) * 8 * G);
+  double *ppr = aux, *ppi = aux + G, *mpr = aux + 2 * G, *mpi = aux + 3 * G, *pmr = aux + 4 * G, *pmi = aux + 5 * G,
+         *mmr = aux + 6 * G, *mmi = aux + 7 * G;
+  for (int r = 0; r < G; ++r) {
+    ppr[r] = cspair[r] * (R(D_SBR)[r] + I(D_SBP)[r]);
+    ppi[r] = cspair[r] * (I(D_SBR)[r] - R(D_SBP)[r]);
+    mpr[r] = -cpair[r] * R(D_RB)[r] - cspair[r] * R(D_SBZ)[r];
+    mpi[r] = -cpair[r] * I(D_RB)[r] - cspair[r] * I(D_SBZ)[r];
+    pmr[r] = cpair[r] * R(D_RB)[r] - cspair[r] * R(D_SBZ)[r];
+    pmi[r] = cpair[r] * I(D_RB)[r] - cspair[r] * I(D_SBZ)[r];
+    mmr[r] = cspair[r] * (-R(D_SBR)[r] + I(D_SBP)[r]);
+    mmi[r] = cspair[r] * (-I(D_SBR)[r] - R(D_SBP)[r]);
+  }
+  const long GN = (long)G * N;
+  double *dpsi = (double *)calloc((size_t)4 * GN, sizeof(double)); /* (sa, ri) */
+#define DP(sa, ri) (dpsi + ((sa) * 2 + (ri)) * GN)
+  for (int b = 0; b < N; ++b) {
+    const double *w = p->wf + (long)b * G;
+    const double *ur, *ui, *dr_, *di;
+    if (p->ns[b] == 1) { ur = ppr; ui = ppi; dr_ = mpr; di = mpi; }
+    else { ur = pmr; ui = pmi; dr_ = mmr; di = mmi; }
+    for (int r = 0; r < G; ++r) {
+      DP(0, 0)[(long)b * G + r] = ur[r] * w[r]; DP(0, 1)[(long)b * G + r] = ui[r] * w[r];
+      DP(1, 0)[(long)b * G + r] = dr_[r] * w[r]; DP(1, 1)[(long)b * G + r] = di[r] * w[r];
+    }
+  }
+  for (int ix = 0; ix < nb; ++ix) {
+    int iy = red->ir2c[ix];
+    if (iy < 0) continue;
+    int ia = p->isstart[ix], ib = p->isstart[iy], nsu = p->num_spin_up[ix], dx = p->db[ix], dy = p->db[iy];
+    if (nsu > 0) {
+      orc_dgemm('t', 'n', nsu, dy, G, 2.0, p->wf + (long)ia * G, G, DP(0, 0) + (long)ib * G, G, 1.0,
+                red->elem + red->ir2m[ix], dx);
+      orc_dgemm('t', 'n', nsu, dy, G, 2.0, p->wf + (long)ia * G, G, DP(0, 1) + (long)ib * G, G, 1.0,
+                imd->elem + imd->ir2m[ix], dx);
+    }
+    if (dx - nsu > 0) {
+      orc_dgemm('t', 'n', dx - nsu, dy, G, 2.0, p->wf + (long)(ia + nsu) * G, G, DP(1, 0) + (long)ib * G, G, 1.0,
+                red->elem + red->ir2m[ix] + nsu, dx);
+      orc_dgemm('t', 'n', dx - nsu, dy, G, 2.0, p->wf + (long)(ia + nsu) * G, G, DP(1, 1) + (long)ib * G, G, 1.0,
+                imd->elem + imd->ir2m[ix] + nsu, dx);
+    }
+  }
+#undef DP
+  free(dpsi);
+  free(aux);
+}
+#undef R
+#undef I
+
+/* calc_hamiltonian: pnfam_hamiltonian_blas.f90:51-74 */
+void orc_calc_hamiltonian(const orc_problem *p, orc_bm *const in[8], orc_bm *const out[8]) {
+  double *ds = (double *)malloc(sizeof(double) * 2 * ND_ * p->nghl);
+  density(p, in[0], in[1], in[2], in[3], ds);
+  meanfield(p, ds, out[0], out[1]);
+  pairingfield(p, ds, out[2], out[3]);
+  density(p, in[4], in[5], in[6], in[7], ds);
+  meanfield(p, ds, out[4], out[5]);
+  pairingfield(p, ds, out[6], out[7]);
+  free(ds);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Modified Broyden mixing: pnfam_broyden.f90:117-216                        */
+/* ------------------------------------------------------------------------ */
+struct orc_broyden_state {
+  long n;
+  int m;
+  double *df, *dv, *curv, *beta, *work, w0;
+};
+
+orc_broyden_state *orc_broyden_new(long n, int m) {
+  orc_broyden_state *s = (orc_broyden_state *)calloc(1, sizeof(*s));
+  s->n = n; s->m = m > 0 ? m : 1;
+  s->df = (double *)calloc((size_t)n * s->m, sizeof(double));
+  s->dv = (double *)calloc((size_t)n * s->m, sizeof(double));
+  s->curv = (double *)calloc((size_t)n, sizeof(double));
+  s->beta = (double *)calloc((size_t)s->m * s->m, sizeof(double));
+  s->work = (double *)calloc((size_t)s->m, sizeof(double));
+  return s;
+}
+void orc_broyden_free(orc_broyden_state *s) {
+  if (!s) return;
+  free(s->df); free(s->dv); free(s->curv); free(s->beta); free(s->work); free(s);
+}
+
+static double ddot(long n, const double *a, const double *b) {
+  double s = 0.0;
+  for (long i = 0; i < n; ++i) s += a[i] * b[i];
+  return s;
+}
+
+/* In-place inverse of a symmetric positive-definite k x k matrix (leading dim m):
+ * Gauss-Jordan with partial pivoting in place of LAPACK DSYTRF/DSYTRI (same
+ * inverse, different rounding). */
+static int sym_inverse(double *a, int k, int m) {
+  double *w = (double *)malloc(sizeof(double) * k * 2 * k);
+  for (int i = 0; i < k; ++i)
+    for (int j = 0; j < 2 * k; ++j) w[i * 2 * k + j] = j < k ? a[i + j * m] : (j - k == i ? 1.0 : 0.0);
+  for (int c = 0; c < k; ++c) {
+    int piv = c;
+    for (int i = c + 1; i < k; ++i)
+      if (fabs(w[i * 2 * k + c]) > fabs(w[piv * 2 * k + c])) piv = i;
+    if (w[piv * 2 * k + c] == 0.0) { free(w); return 1; }
+    if (piv != c)
+      for (int j = 0; j < 2 * k; ++j) { double t = w[c * 2 * k + j]; w[c * 2 * k + j] = w[piv * 2 * k + j]; w[piv * 2 * k + j] = t; }
+    double d = 1.0 / w[c * 2 * k + c];
+    for (int j = 0; j < 2 * k; ++j) w[c * 2 * k + j] *= d;
+    for (int i = 0; i < k; ++i) {
+      if (i == c) continue;
+      double f = w[i * 2 * k + c];
+      if (f == 0.0) continue;
+      for (int j = 0; j < 2 * k; ++j) w[i * 2 * k + j] -= f * w[c * 2 * k + j];
+    }
+  }
+  for (int i = 0; i < k; ++i)
+    for (int j = 0; j < k; ++j) a[i + j * m] = w[i * 2 * k + k + j];
+  free(w);
+  return 0;
+}
+
+double orc_broyden(orc_broyden_state *st, long n, double *vout, double *vin, double alpha, int iter, int M,
+                   char *bbroyden) {
+  double si = 0.0;
+  for (long i = 0; i < n; ++i) { vout[i] -= vin[i]; if (fabs(vout[i]) > si) si = fabs(vout[i]); }
+  if (M < 0) { *bbroyden = 'N'; for (long i = 0; i < n; ++i) vin[i] += vout[i]; return si; }
+  if (M == 0 || iter == 0) { *bbroyden = 'L'; for (long i = 0; i < n; ++i) vin[i] += alpha * vout[i]; return si; }
+  *bbroyden = 'B';
+  int iter_used = iter - 1 < M ? iter - 1 : M;
+  int ipos = iter - 1 - ((iter - 2) / M) * M; /* Fortran integer division truncates */
+  int inext = iter - ((iter - 1) / M) * M;
+  double *df = st->df, *dv = st->dv, *beta = st->beta, *work = st->work, *curv = st->curv;
+  if (iter == 1) {
+    st->w0 = 0.010;
+  } else {
+    double *dfp = df + (long)(ipos - 1) * n, *dvp = dv + (long)(ipos - 1) * n;
+    for (long i = 0; i < n; ++i) { dfp[i] = vout[i] - dfp[i]; dvp[i] = vin[i] - dvp[i]; }
+    double nrm = sqrt(ddot(n, dfp, dfp));
+    double normi = 1.0 / sqrt(nrm * nrm);
+    for (long i = 0; i < n; ++i) { dfp[i] *= normi; dvp[i] *= normi; }
+  }
+  for (int i = 0; i < iter_used; ++i) {
+    for (int j = i + 1; j < iter_used; ++j) beta[i + j * M] = ddot(n, df + (long)j * n, df + (long)i * n);
+    beta[i + i * M] = st->w0 * st->w0 + 1.0;
+  }
+  for (int i = 0; i < iter_used; ++i)
+    for (int j = i + 1; j < iter_used; ++j) beta[j + i * M] = beta[i + j * M];
+  if (iter_used > 0 && sym_inverse(beta, iter_used, M)) return -1.0;
+  for (int i = 0; i < iter_used; ++i) work[i] = ddot(n, df + (long)i * n, vout);
+  for (long k = 0; k < n; ++k) curv[k] = alpha * vout[k];
+  for (int i = 0; i < iter_used; ++i) {
+    double gamma = 0.0;
+    for (int j = 0; j < iter_used; ++j) gamma += beta[j + i * M] * work[j];
+    const double *dvi = dv + (long)i * n, *dfi = df + (long)i * n;
+    for (long k = 0; k < n; ++k) curv[k] -= gamma * (dvi[k] + alpha * dfi[k]);
+  }
+  memcpy(df + (long)(inext - 1) * n, vout, sizeof(double) * n);
+  memcpy(dv + (long)(inext - 1) * n, vin, sizeof(double) * n);
+  for (long k = 0; k < n; ++k) vin[k] += curv[k];
+  return si;
+}
+
+/* ------------------------------------------------------------------------ */
+/* The pnFAM solve: init_pnfam_solver + ifam (pnfam_solver.f90)              */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+  int nb; const int *db; long nxy;
+  orc_bm *Up, *Vp, *Un, *Vn, *fsp;
+  bbm Wp, Wn, Fsp, Fqp, dHqp_re, dHqp_im, dRqp_re, dRqp_im, dHsp_re, dHsp_im, dRsp_re, dRsp_im;
+  orc_bm *G_re12, *G_im12, *G_re21, *G_im21;
+} solver;
+
+static void solver_init(solver *S, const orc_problem *p) {
+  int nb = p->nb;
+  S->nb = nb; S->db = p->db;
+  long dmat = 0;
+  for (int i = 0; i < nb; ++i) dmat += (long)p->db[i] * p->db[i];
+  int *diag = (int *)malloc(sizeof(int) * nb), *tr = (int *)malloc(sizeof(int) * nb);
+  for (int i = 0; i < nb; ++i) { diag[i] = i; tr[i] = i < nb / 2 ? i + nb / 2 : i - nb / 2; }
+  S->Up = bm_new(nb, dmat); S->Vp = bm_new(nb, dmat); S->Un = bm_new(nb, dmat); S->Vn = bm_new(nb, dmat);
+  bm_set_map(nb, p->db, S->Up, diag); bm_set_map(nb, p->db, S->Un, diag);
+  bm_set_map(nb, p->db, S->Vp, tr); bm_set_map(nb, p->db, S->Vn, tr);
+  /* V storage follows U's offsets (pnfam_setup.f90:314-320) */
+  for (int i = 0; i < nb; ++i) { S->Vp->ir2m[i] = S->Vn->ir2m[i] = S->Up->ir2m[i]; }
+  for (int i = 0; i < nb; ++i) { S->Vp->ic2m[S->Vp->ir2c[i]] = S->Vp->ir2m[i]; S->Vn->ic2m[S->Vn->ir2c[i]] = S->Vn->ir2m[i]; }
+  memcpy(S->Up->elem, p->Up, sizeof(double) * dmat); memcpy(S->Vp->elem, p->Vp, sizeof(double) * dmat);
+  memcpy(S->Un->elem, p->Un, sizeof(double) * dmat); memcpy(S->Vn->elem, p->Vn, sizeof(double) * dmat);
+  long nxy = 0;
+  for (int i = 0; i < nb; ++i) if (p->f_ir2c[i] >= 0) nxy += (long)p->db[i] * p->db[p->f_ir2c[i]];
+  S->nxy = nxy;
+  S->fsp = bm_new(nb, nxy); bm_set_map(nb, p->db, S->fsp, p->f_ir2c);
+  memcpy(S->fsp->elem, p->f_elem, sizeof(double) * nxy);
+  free(diag); free(tr);
+  /* W = [U V; V U] */
+  S->Wp = bbm_init(); S->Wp.m[0] = S->Up; S->Wp.m[1] = S->Vp; S->Wp.m[2] = S->Vp; S->Wp.m[3] = S->Up;
+  S->Wn = bbm_init(); S->Wn.m[0] = S->Un; S->Wn.m[1] = S->Vn; S->Wn.m[2] = S->Vn; S->Wn.m[3] = S->Un;
+  S->Fsp = bbm_init(); S->Fsp.m[0] = S->fsp;
+  /* qp quantities: m12, m21 with signs (+,+,-,-) */
+  bbm *qp[5] = {&S->Fqp, &S->dHqp_re, &S->dHqp_im, &S->dRqp_re, &S->dRqp_im};
+  for (int k = 0; k < 5; ++k) {
+    *qp[k] = bbm_init();
+    qp[k]->m[1] = bm_new(nb, nxy); qp[k]->m[2] = bm_new(nb, nxy);
+    qp[k]->s[2] = -1.0; qp[k]->s[3] = -1.0;
+  }
+  S->dHqp_im.imag = 1; S->dRqp_im.imag = 1;
+  bbm *sp[4] = {&S->dHsp_re, &S->dHsp_im, &S->dRsp_re, &S->dRsp_im};
+  for (int k = 0; k < 4; ++k) {
+    *sp[k] = bbm_init();
+    for (int q = 0; q < 4; ++q) sp[k]->m[q] = bm_new(nb, nxy);
+  }
+  /* dHsp: re signs (+,+,-,-), im (+,+,+,-), trans (n,n,n,t) */
+  S->dHsp_re.s[2] = -1; S->dHsp_re.s[3] = -1; S->dHsp_re.t[3] = 't';
+  S->dHsp_im.s[3] = -1; S->dHsp_im.t[3] = 't'; S->dHsp_im.imag = 1;
+  /* dRsp: re signs (+,-,+,-), im (+,-,-,-), trans (n,t,t,t) */
+  S->dRsp_re.s[1] = -1; S->dRsp_re.s[3] = -1;
+  S->dRsp_im.s[1] = -1; S->dRsp_im.s[2] = -1; S->dRsp_im.s[3] = -1; S->dRsp_im.imag = 1;
+  for (int q = 1; q < 4; ++q) { S->dRsp_re.t[q] = 't'; S->dRsp_im.t[q] = 't'; }
+  /* Fqp = Wp^T Fsp Wn (beta-minus) */
+  if (p->bminus) triprod_bbm(nb, p->db, 't', S->Wp, 'n', S->Fsp, 'n', S->Wn, &S->Fqp);
+  else triprod_bbm(nb, p->db, 't', S->Wn, 'n', S->Fsp, 'n', S->Wp, &S->Fqp);
+  bbm *cp[4] = {&S->dRqp_re, &S->dRqp_im, &S->dHqp_re, &S->dHqp_im};
+  for (int k = 0; k < 4; ++k) {
+    bm_copy_structure(nb, S->Fqp.m[1], cp[k]->m[1]);
+    bm_copy_structure(nb, S->Fqp.m[2], cp[k]->m[2]);
+  }
+  /* dHsp structures (pnfam_solver.f90:402-413) */
+  for (int k = 0; k < 2; ++k) {
+    bbm *h = k ? &S->dHsp_im : &S->dHsp_re;
+    bm_copy_structure(nb, S->fsp, h->m[0]);
+    bm_copy_structure(nb, S->dRqp_re.m[1], h->m[1]);
+    bm_copy_structure(nb, S->dRqp_re.m[2], h->m[2]);
+    bm_copy_structure(nb, S->fsp, h->m[3]);
+  }
+  S->G_re12 = bm_new(nb, nxy); S->G_im12 = bm_new(nb, nxy); S->G_re21 = bm_new(nb, nxy); S->G_im21 = bm_new(nb, nxy);
+  bm_copy_structure(nb, S->Fqp.m[1], S->G_re12); bm_copy_structure(nb, S->Fqp.m[1], S->G_im12);
+  bm_copy_structure(nb, S->Fqp.m[2], S->G_re21); bm_copy_structure(nb, S->Fqp.m[2], S->G_im21);
+}
+
+static void solver_free(solver *S) {
+  bm_free(S->Up); bm_free(S->Vp); bm_free(S->Un); bm_free(S->Vn); bm_free(S->fsp);
+  bbm *qp[5] = {&S->Fqp, &S->dHqp_re, &S->dHqp_im, &S->dRqp_re, &S->dRqp_im};
+  for (int k = 0; k < 5; ++k) { bm_free(qp[k]->m[1]); bm_free(qp[k]->m[2]); }
+  bbm *sp[4] = {&S->dHsp_re, &S->dHsp_im, &S->dRsp_re, &S->dRsp_im};
+  for (int k = 0; k < 4; ++k) for (int q = 0; q < 4; ++q) bm_free(sp[k]->m[q]);
+  bm_free(S->G_re12); bm_free(S->G_im12); bm_free(S->G_re21); bm_free(S->G_im21);
+}
+
+/* matrix_2qp: pnfam_solver.f90:510-544, M_ij = a (b f1_i + c f2_j + e)^d with d = -1 */
+static void matrix_2qp(int nb, const int *db, const int *isstart, int a, int b, int c, double ree, double ime,
+                       int want_re, const double *f1, const double *f2, orc_bm *mat) {
+  long ipt = 0;
+  for (int ibr = 0; ibr < nb; ++ibr) {
+    int ibc = mat->ir2c[ibr];
+    if (ibc < 0) continue;
+    for (int i2 = isstart[ibc]; i2 < isstart[ibc] + db[ibc]; ++i2)
+      for (int i1 = isstart[ibr]; i1 < isstart[ibr] + db[ibr]; ++i1) {
+        double xr = b * f1[i1] + c * f2[i2] + ree, xi = ime;
+        double den = xr * xr + xi * xi; /* (x)^-1 = conj(x)/|x|^2 */
+        double zr = a * xr / den, zi = -a * xi / den;
+        mat->elem[ipt++] = want_re ? zr : zi;
+      }
+  }
+}
+
+int orc_ifam(const orc_problem *p, const orc_opts *o, double re_w, double im_w, double *re_s, double *im_s,
+             double *trace, char *trace_flag, int *n_trace, double *x_out) {
+  solver S;
+  solver_init(&S, p);
+  const int nb = p->nb;
+  const long nxy = S.nxy;
+  const double *f1 = p->bminus ? p->Ep : p->En, *f2 = p->bminus ? p->En : p->Ep;
+  matrix_2qp(nb, p->db, p->isstart, -1, +1, +1, -re_w, -im_w, 1, f1, f2, S.G_re12);
+  matrix_2qp(nb, p->db, p->isstart, -1, +1, +1, -re_w, -im_w, 0, f1, f2, S.G_im12);
+  matrix_2qp(nb, p->db, p->isstart, -1, +1, +1, +re_w, +im_w, 1, f1, f2, S.G_re21);
+  matrix_2qp(nb, p->db, p->isstart, -1, +1, +1, +re_w, +im_w, 0, f1, f2, S.G_im21);
+  int M = o->broyden_history;
+  if (fabs(o->quench) < 1e-10) M = -1;
+  const long nbroy = 4 * nxy;
+  double *broin = (double *)calloc((size_t)nbroy, sizeof(double));
+  double *broout = (double *)calloc((size_t)nbroy, sizeof(double));
+  orc_broyden_state *bst = orc_broyden_new(nbroy, M > 0 ? M : 1);
+  double si = 1.0, rs = 0.0, is = 0.0;
+  char bflag = 'N';
+  int iter, conv = -1;
+  *n_trace = 0;
+  for (iter = 0; iter <= o->max_iter; ++iter) {
+    trace[3 * iter] = si; trace[3 * iter + 1] = rs; trace[3 * iter + 2] = is;
+    trace_flag[iter] = bflag; *n_trace = iter + 1;
+    if (si < o->eps) { conv = iter; break; }
+    if (iter == o->max_iter) { conv = -1; break; }
+    if (fabs(o->quench) < 1e-10) {
+      memset(S.dHqp_re.m[1]->elem, 0, sizeof(double) * nxy); memset(S.dHqp_re.m[2]->elem, 0, sizeof(double) * nxy);
+      memset(S.dHqp_im.m[1]->elem, 0, sizeof(double) * nxy); memset(S.dHqp_im.m[2]->elem, 0, sizeof(double) * nxy);
+    } else {
+      if (p->bminus) {
+        triprod_bbm(nb, p->db, 'n', S.Wp, 'n', S.dRqp_re, 't', S.Wn, &S.dRsp_re);
+        triprod_bbm(nb, p->db, 'n', S.Wp, 'n', S.dRqp_im, 't', S.Wn, &S.dRsp_im);
+      } else {
+        triprod_bbm(nb, p->db, 'n', S.Wn, 'n', S.dRqp_re, 't', S.Wp, &S.dRsp_re);
+        triprod_bbm(nb, p->db, 'n', S.Wn, 'n', S.dRqp_im, 't', S.Wp, &S.dRsp_im);
+      }
+      orc_bm *in[8] = {S.dRsp_re.m[0], S.dRsp_im.m[0], S.dRsp_re.m[1], S.dRsp_im.m[1],
+                       S.dRsp_re.m[3], S.dRsp_im.m[3], S.dRsp_re.m[2], S.dRsp_im.m[2]};
+      orc_bm *out[8] = {S.dHsp_re.m[0], S.dHsp_im.m[0], S.dHsp_re.m[1], S.dHsp_im.m[1],
+                        S.dHsp_re.m[3], S.dHsp_im.m[3], S.dHsp_re.m[2], S.dHsp_im.m[2]};
+      orc_calc_hamiltonian(p, in, out);
+      if (p->bminus) {
+        triprod_bbm(nb, p->db, 't', S.Wp, 'n', S.dHsp_re, 'n', S.Wn, &S.dHqp_re);
+        triprod_bbm(nb, p->db, 't', S.Wp, 'n', S.dHsp_im, 'n', S.Wn, &S.dHqp_im);
+      } else {
+        triprod_bbm(nb, p->db, 't', S.Wn, 'n', S.dHsp_re, 'n', S.Wp, &S.dHqp_re);
+        triprod_bbm(nb, p->db, 't', S.Wn, 'n', S.dHsp_im, 'n', S.Wp, &S.dHqp_im);
+      }
+      for (int q = 1; q <= 2; ++q)
+        for (long i = 0; i < nxy; ++i) { S.dHqp_re.m[q]->elem[i] *= o->quench; S.dHqp_im.m[q]->elem[i] *= o->quench; }
+    }
+    /* add external field; X = gX (F20 + H20), Y = gY (F02 + H02) */
+    for (long i = 0; i < nxy; ++i) {
+      S.dHqp_re.m[1]->elem[i] += S.Fqp.m[1]->elem[i];
+      S.dHqp_re.m[2]->elem[i] += S.Fqp.m[2]->elem[i];
+    }
+    for (long i = 0; i < nxy; ++i) {
+      double gr = S.G_re12->elem[i], gi = S.G_im12->elem[i];
+      double hr = S.dHqp_re.m[1]->elem[i], hi = S.dHqp_im.m[1]->elem[i];
+      S.dRqp_re.m[1]->elem[i] = gr * hr - gi * hi; S.dRqp_im.m[1]->elem[i] = gi * hr + hi * gr;
+      gr = S.G_re21->elem[i]; gi = S.G_im21->elem[i];
+      hr = S.dHqp_re.m[2]->elem[i]; hi = S.dHqp_im.m[2]->elem[i];
+      S.dRqp_re.m[2]->elem[i] = gr * hr - gi * hi; S.dRqp_im.m[2]->elem[i] = gi * hr + hi * gr;
+    }
+    /* Broyden on [ReX, ReY, ImX, ImY] */
+    memcpy(broout, S.dRqp_re.m[1]->elem, sizeof(double) * nxy);
+    memcpy(broout + nxy, S.dRqp_re.m[2]->elem, sizeof(double) * nxy);
+    memcpy(broout + 2 * nxy, S.dRqp_im.m[1]->elem, sizeof(double) * nxy);
+    memcpy(broout + 3 * nxy, S.dRqp_im.m[2]->elem, sizeof(double) * nxy);
+    si = orc_broyden(bst, nbroy, broout, broin, o->alpha_mix, iter, M, &bflag);
+    memcpy(S.dRqp_re.m[1]->elem, broin, sizeof(double) * nxy);
+    memcpy(S.dRqp_re.m[2]->elem, broin + nxy, sizeof(double) * nxy);
+    memcpy(S.dRqp_im.m[1]->elem, broin + 2 * nxy, sizeof(double) * nxy);
+    memcpy(S.dRqp_im.m[2]->elem, broin + 3 * nxy, sizeof(double) * nxy);
+    rs = -contract_bbm(&S.Fqp, &S.dRqp_re) / PI;
+    is = -contract_bbm(&S.Fqp, &S.dRqp_im) / PI;
+  }
+  *re_s = rs; *im_s = is;
+  if (x_out) memcpy(x_out, broin, sizeof(double) * nbroy);
+  orc_broyden_free(bst);
+  free(broin); free(broout);
+  solver_free(&S);
+  return conv;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Per-unit hooks (the same bbm products as ifam)                            */
+/* ------------------------------------------------------------------------ */
+void orc_fqp(const orc_problem *p, double *f20, double *f02, int *x_ir2c, int *y_ir2c) {
+  solver S;
+  solver_init(&S, p);
+  memcpy(f20, S.Fqp.m[1]->elem, sizeof(double) * S.nxy);
+  memcpy(f02, S.Fqp.m[2]->elem, sizeof(double) * S.nxy);
+  memcpy(x_ir2c, S.Fqp.m[1]->ir2c, sizeof(int) * p->nb);
+  memcpy(y_ir2c, S.Fqp.m[2]->ir2c, sizeof(int) * p->nb);
+  solver_free(&S);
+}
+
+void orc_qp_to_sp(const orc_problem *p, const double *x_re, const double *x_im, const double *y_re,
+                  const double *y_im, double *rsp, int *maps) {
+  solver S;
+  solver_init(&S, p);
+  const long nxy = S.nxy;
+  memcpy(S.dRqp_re.m[1]->elem, x_re, sizeof(double) * nxy); memcpy(S.dRqp_im.m[1]->elem, x_im, sizeof(double) * nxy);
+  memcpy(S.dRqp_re.m[2]->elem, y_re, sizeof(double) * nxy); memcpy(S.dRqp_im.m[2]->elem, y_im, sizeof(double) * nxy);
+  if (p->bminus) {
+    triprod_bbm(p->nb, p->db, 'n', S.Wp, 'n', S.dRqp_re, 't', S.Wn, &S.dRsp_re);
+    triprod_bbm(p->nb, p->db, 'n', S.Wp, 'n', S.dRqp_im, 't', S.Wn, &S.dRsp_im);
+  } else {
+    triprod_bbm(p->nb, p->db, 'n', S.Wn, 'n', S.dRqp_re, 't', S.Wp, &S.dRsp_re);
+    triprod_bbm(p->nb, p->db, 'n', S.Wn, 'n', S.dRqp_im, 't', S.Wp, &S.dRsp_im);
+  }
+  for (int q = 0; q < 4; ++q) {
+    memcpy(rsp + (long)(2 * q) * nxy, S.dRsp_re.m[q]->elem, sizeof(double) * nxy);
+    memcpy(rsp + (long)(2 * q + 1) * nxy, S.dRsp_im.m[q]->elem, sizeof(double) * nxy);
+    memcpy(maps + q * p->nb, S.dRsp_re.m[q]->ir2c, sizeof(int) * p->nb);
+  }
+  solver_free(&S);
+}
+
+void orc_sp_to_qp(const orc_problem *p, const double *hsp, const int *maps, double *h20_re, double *h20_im,
+                  double *h02_re, double *h02_im) {
+  solver S;
+  solver_init(&S, p);
+  const long nxy = S.nxy;
+  for (int q = 0; q < 4; ++q) {
+    bm_set_map(p->nb, p->db, S.dHsp_re.m[q], maps + q * p->nb);
+    bm_set_map(p->nb, p->db, S.dHsp_im.m[q], maps + q * p->nb);
+    memcpy(S.dHsp_re.m[q]->elem, hsp + (long)(2 * q) * nxy, sizeof(double) * nxy);
+    memcpy(S.dHsp_im.m[q]->elem, hsp + (long)(2 * q + 1) * nxy, sizeof(double) * nxy);
+  }
+  if (p->bminus) {
+    triprod_bbm(p->nb, p->db, 't', S.Wp, 'n', S.dHsp_re, 'n', S.Wn, &S.dHqp_re);
+    triprod_bbm(p->nb, p->db, 't', S.Wp, 'n', S.dHsp_im, 'n', S.Wn, &S.dHqp_im);
+  } else {
+    triprod_bbm(p->nb, p->db, 't', S.Wn, 'n', S.dHsp_re, 'n', S.Wp, &S.dHqp_re);
+    triprod_bbm(p->nb, p->db, 't', S.Wn, 'n', S.dHsp_im, 'n', S.Wp, &S.dHqp_im);
+  }
+  memcpy(h20_re, S.dHqp_re.m[1]->elem, sizeof(double) * nxy); memcpy(h20_im, S.dHqp_im.m[1]->elem, sizeof(double) * nxy);
+  memcpy(h02_re, S.dHqp_re.m[2]->elem, sizeof(double) * nxy); memcpy(h02_im, S.dHqp_im.m[2]->elem, sizeof(double) * nxy);
+  solver_free(&S);
+}

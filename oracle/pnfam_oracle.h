@@ -1,0 +1,87 @@
+/*
+ * CPU oracle for the pnFAM iteration -- TEST INFRASTRUCTURE ONLY.
+ *
+ * A plain-C, FP64 restatement of the reference's hot path, used by tests/ as
+ * the checker and by bench.py as the CPU baseline.  Nothing in the product
+ * (pynfam_amd/, include/) may link, load or call it.
+ *
+ * Functions and the reference code each follows:
+ *   orc_dgemm            column-major GEMM standing in for BLAS dgemm
+ *   orc_triprod          pnfam_type_blockmatrix.f90:140-211 (triprod)
+ *   orc_triprod_bbm      pnfam_type_bbm.f90:428-576 (triprod_bbm_quad / triprod_bbm)
+ *   orc_calc_hamiltonian pnfam_hamiltonian_blas.f90:51-74, 124-711, 717-1169, 1175-1262
+ *   orc_broyden          pnfam_broyden.f90:117-216 (modified Broyden, Johnson 1988)
+ *   orc_ifam             pnfam_solver.f90:93-221 (ifam) with init_pnfam_solver :226-460
+ *                        and matrix_2qp :510-544
+ *
+ * Block maps are 0-based with -1 for "no block"; element storage follows
+ * pnfam_type_blockmatrix.f90:19-26.
+ */
+#ifndef PNFAM_ORACLE_H
+#define PNFAM_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+  int *ir2c, *ic2r, *ir2m, *ic2m; /* nb each */
+  double *elem;
+  long n;
+} orc_bm;
+
+/* Problem description: the ω-independent inputs of one (operator, K) solve. */
+typedef struct {
+  int nb, dqp, nghl;
+  const int *db, *isstart, *num_spin_up, *ns; /* ns: per state (+1/-1) */
+  const double *wf, *wfdr, *wfdz, *wfdp, *wfd2_all; /* (nghl x dqp) column-major */
+  const double *wdcori;                            /* nghl */
+  const double *Ep, *En;                           /* dqp */
+  const double *Up, *Vp, *Un, *Vn;                 /* sum(db^2) each */
+  const int *f_ir2c;                               /* operator block map (nb) */
+  const double *f_elem;                            /* operator elements */
+  const double *crho, *cs, *cpair, *cspair;        /* nghl each */
+  const double *cscal; /* cdrho ctau ctj0 ctj1 ctj2 crdj cds ct cj cgs cf csdj */
+  int bminus;
+} orc_problem;
+
+typedef struct {
+  int max_iter;
+  double eps;
+  int broyden_history;
+  double quench;
+  double alpha_mix;
+} orc_opts;
+
+void orc_dgemm(char ta, char tb, int m, int n, int k, double alpha, const double *A, int lda,
+               const double *B, int ldb, double beta, double *C, int ldc);
+
+/* 16-array calc_hamiltonian with the semantics of calc_dHsp_interface
+ * (pnfam_setup.f90:56-71).  in[8]: re/im rho_pn, kappa+, rho_np, kappa-;
+ * out[8]: re/im h_pn, Delta+, h_np, Delta-.  Block maps per array. */
+void orc_calc_hamiltonian(const orc_problem *p, orc_bm *const in[8], orc_bm *const out[8]);
+
+/* One modified-Broyden step on a real vector (history kept in a handle). */
+typedef struct orc_broyden_state orc_broyden_state;
+orc_broyden_state *orc_broyden_new(long n, int m);
+void orc_broyden_free(orc_broyden_state *st);
+double orc_broyden(orc_broyden_state *st, long n, double *vout, double *vin, double alpha, int iter,
+                   int m, char *bbroyden);
+
+/* Full solve at one complex frequency.  trace: (max_iter+1) x 3 (si, ReS, ImS)
+ * per printed iteration line; returns iterations (>=0 converged, -1 not). */
+int orc_ifam(const orc_problem *p, const orc_opts *o, double re_w, double im_w, double *re_s,
+             double *im_s, double *trace, char *trace_flag, int *n_trace, double *x_out);
+
+/* Individual hot-path units for per-call parity (X,Y stored as in Fqp). */
+void orc_qp_to_sp(const orc_problem *p, const double *x_re, const double *x_im, const double *y_re,
+                  const double *y_im, double *rsp /* 8 * nxy: re/im m11 m12 m21 m22 */,
+                  int *maps /* 4 * nb ir2c */);
+void orc_sp_to_qp(const orc_problem *p, const double *hsp /* 8*nxy */, const int *maps,
+                  double *h20_re, double *h20_im, double *h02_re, double *h02_im);
+void orc_fqp(const orc_problem *p, double *f20, double *f02, int *x_ir2c, int *y_ir2c);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,171 @@
+"""Extract the golden fixtures for the pnFAM iteration from the reference's archived runs.
+
+The reference ships complete pynfam output trees for one-body GT- runs at N_sh=6
+(tests/S40 closed tests/pynfam_test_GT-S40, tests/Gd162 closed tests/GT).  Each holds
+the HFB input (hfbtho_NAMELIST.dat + hfbtho_output.hel, read by pnFAM through
+hfbtho_interface.f90:20-226) and, inside fam_soln/fam_meta/OP-K.tar, one
+pnFAM namelist (.in) and log (.dat) per contour point.  The .dat logs carry the
+19-digit strength (pnfam_txtoutput.f90:230-253), the per-iteration si / Re S / Im S
+trace (pnfam_solver.f90:109-118) and the coupling constants.
+
+This script only reads data files (text, tar, Fortran-unformatted binary copied
+byte for byte); it imports and runs nothing from the reference.  Output goes to
+tests/golden/<set>/ as small JSON files plus the two HFB input files.
+
+Usage:  python tests/golden/make_fixtures.py   (needs /root/reference)
+"""
+import io
+import json
+import os
+import re
+import shutil
+import sys
+import tarfile
+
+REF = "/root/reference"
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+SETS = {
+    "S40": "tests/S40 closed tests/pynfam_test_GT-S40/000000",
+    "Gd162": "tests/Gd162 closed tests/GT/000000",
+}
+
+FLOAT = r"[-+]?\d+\.\d+(?:[EeDd][-+]?\d+)?"
+
+
+def _f(s):
+    return float(s.replace("D", "E").replace("d", "e"))
+
+
+def parse_nml_text(txt):
+    """Tiny namelist scraper for the scalar entries of a pnFAM .in file."""
+    out = {}
+    for line in txt.splitlines():
+        m = re.match(r"\s*(\w+)\s*=\s*(.*?)\s*$", line)
+        if not m:
+            continue
+        key, val = m.group(1).lower(), m.group(2).rstrip(",")
+        if val.startswith("'"):
+            out[key] = val.strip("'")
+        elif val in (".true.", ".false."):
+            out[key] = val == ".true."
+        else:
+            try:
+                out[key] = int(val)
+            except ValueError:
+                try:
+                    out[key] = _f(val)
+                except ValueError:
+                    out[key] = val
+    return out
+
+
+def parse_dat(txt):
+    """Parse one pnFAM .dat log: trace, convergence line, result footer, couplings."""
+    rec = {"trace": []}
+    for line in txt.splitlines():
+        m = re.match(r"#\s+(\d+)([NLB])\s+(" + FLOAT + r")\s+(" + FLOAT + r")\s+(" + FLOAT + r")\s+(" + FLOAT + r")", line)
+        if m:
+            rec["trace"].append([int(m.group(1)), m.group(2), _f(m.group(3)), _f(m.group(4)),
+                                 _f(m.group(5))])
+            continue
+        m = re.search(r"FAM iteration (converged|interrupted) after\s+(\d+) steps\.\s+si =\s+(\S+)", line)
+        if m:
+            rec["converged"] = m.group(1) == "converged"
+            rec["iterations"] = int(m.group(2))
+            rec["si_final"] = _f(m.group(3))
+            continue
+        m = re.match(r"\s+Energy\s+(\S+)\s+(\S+)", line)
+        if m:
+            rec["energy"] = [_f(m.group(1)), _f(m.group(2))]
+            continue
+        m = re.match(r"\s+Strength\s+(\S+)\s+(\S+)", line)
+        if m:
+            rec["strength"] = [_f(m.group(1)), _f(m.group(2))]
+            continue
+        m = re.match(r"# Number shells:\s+(\d+)", line)
+        if m:
+            rec["n_shells"] = int(m.group(1))
+        m = re.match(r"# Basis size:\s+(\d+)", line)
+        if m:
+            rec["basis_size"] = int(m.group(1))
+        m = re.match(r"# Number matrix blocks:\s+(\d+)", line)
+        if m:
+            rec["n_blocks"] = int(m.group(1))
+        m = re.match(r"# Non-trivial FAM matrix elements:\s+(\d+)", line)
+        if m:
+            rec["nxy"] = int(m.group(1))
+    cpl = {}
+    in_natural = False
+    for line in txt.splitlines():
+        if "natural units" in line:
+            in_natural = True
+        if in_natural:
+            continue
+        for name, val in re.findall(r"(C\w+(?:\[\w\])?|\(sigma\))\s*=\s*(" + FLOAT + ")", line):
+            key = name
+            while key in cpl:
+                key += "'"
+            cpl[key] = _f(val)
+    rec["couplings_printed"] = cpl
+    return rec
+
+
+def parse_thoout(txt):
+    """Quasiparticle energies per block and Fermi energies from HFBTHO's thoout.dat."""
+    out = {}
+    blocks = txt.split("#quasiparticle energies")
+    for part, label in zip(blocks[1:3], ("neutrons", "protons")):
+        m = re.search(r"fermi energy alast=\s*(\S+)", part)
+        alast = _f(m.group(1))
+        eqp = []
+        for line in part.splitlines():
+            m = re.match(r"\s+(\d+)\s+(\d+)\s+(" + FLOAT + r")\s+(" + FLOAT + r")\s+(" + FLOAT + r")", line)
+            if m:
+                eqp.append([int(m.group(2)), _f(m.group(3)), _f(m.group(4))])
+        out[label] = {"alast": alast, "eqp_block_e": eqp}
+    return out
+
+
+def extract_set(name, rel):
+    src = os.path.join(REF, rel)
+    dst = os.path.join(HERE, name)
+    os.makedirs(dst, exist_ok=True)
+    for fn in ("hfbtho_NAMELIST.dat", "hfbtho_output.hel"):
+        shutil.copyfile(os.path.join(src, "hfb_soln", fn), os.path.join(dst, fn))
+    with open(os.path.join(src, "hfb_soln", "thoout.dat")) as fh:
+        tho = parse_thoout(fh.read())
+    with open(os.path.join(dst, "thoout_qp.json"), "w") as fh:
+        json.dump(tho, fh, indent=1)
+    meta = os.path.join(src, "fam_soln", "fam_meta")
+    for tname in sorted(os.listdir(meta)):
+        if not tname.endswith(".tar"):
+            continue
+        op = tname[:-4]
+        points = {}
+        with tarfile.open(os.path.join(meta, tname)) as tf:
+            for member in tf.getmembers():
+                if not member.isfile():
+                    continue
+                pid = member.name.split("/")[1]
+                txt = tf.extractfile(member).read().decode()
+                d = points.setdefault(pid, {})
+                if member.name.endswith(".in"):
+                    d["namelist"] = parse_nml_text(txt)
+                elif member.name.endswith(".dat"):
+                    d.update(parse_dat(txt))
+        rows = [dict(point=k, **v) for k, v in sorted(points.items())]
+        with open(os.path.join(dst, op + ".json"), "w") as fh:
+            json.dump(rows, fh, indent=0)
+        print(name, op, len(rows), "points")
+
+
+def main():
+    if not os.path.isdir(REF):
+        sys.exit("reference tree not present")
+    for name, rel in SETS.items():
+        extract_set(name, rel)
+
+
+if __name__ == "__main__":
+    main()
