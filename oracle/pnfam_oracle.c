@@ -109,6 +109,12 @@ static void triprod(int nb, const int *db, char ta, const orc_bm *a, char tb, co
   free(aux);
 }
 
+/* exported for the reference's module KAT (tests/modules/blockmatrix_type_test.f90:68-95) */
+void orc_triprod(int nb, const int *db, char ta, const orc_bm *a, char tb, const orc_bm *b, char tc, const orc_bm *c,
+                 double alpha, double beta, orc_bm *abc) {
+  triprod(nb, db, ta, a, tb, b, tc, c, alpha, beta, abc);
+}
+
 /* 2x2 quadrant matrix with per-quadrant transpose/sign flags (pnfam_type_bbm.f90:30-35) */
 typedef struct {
   orc_bm *m[4]; /* 11 12 21 22 */
@@ -765,7 +771,7 @@ static void solver_init(solver *S, const orc_problem *p) {
   S->nb = nb; S->db = p->db;
   long dmat = 0;
   for (int i = 0; i < nb; ++i) dmat += (long)p->db[i] * p->db[i];
-  int *diag = (int *)malloc(sizeof(int) * nb), *tr = (int *)malloc(sizeof(int) * nb);
+  int *diag = (int *)calloc(nb > 0 ? nb : 1, sizeof(int)), *tr = (int *)calloc(nb > 0 ? nb : 1, sizeof(int));
   for (int i = 0; i < nb; ++i) { diag[i] = i; tr[i] = i < nb / 2 ? i + nb / 2 : i - nb / 2; }
   S->Up = bm_new(nb, dmat); S->Vp = bm_new(nb, dmat); S->Un = bm_new(nb, dmat); S->Vn = bm_new(nb, dmat);
   bm_set_map(nb, p->db, S->Up, diag); bm_set_map(nb, p->db, S->Un, diag);

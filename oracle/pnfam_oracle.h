@@ -53,6 +53,8 @@ typedef struct {
   double alpha_mix;
 } orc_opts;
 
+void orc_triprod(int nb, const int *db, char ta, const orc_bm *a, char tb, const orc_bm *b, char tc, const orc_bm *c,
+                 double alpha, double beta, orc_bm *abc);
 void orc_dgemm(char ta, char tb, int m, int n, int k, double alpha, const double *A, int lda,
                const double *B, int ldb, double beta, double *C, int ldc);
 

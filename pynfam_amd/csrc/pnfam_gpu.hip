@@ -1,0 +1,1377 @@
+// pnFAM iteration engine for MI355X (gfx950): kernels + C ABI (include/pnfam_gpu.h).
+//
+// Per-frequency state lives in HBM as frequency-major arrays:
+//   vin/vout [w][4][nxy]  mixed / new amplitudes  (ReX, ReY, ImX, ImY) -- the
+//                         Broyden vector layout of pnfam_broyden.f90:50-53
+//   rsp      [w][8][nxy]  drho_sp quadrants re/im (m11, m12, m21, m22)
+//   hsp      [w][8][nxy]  dH_sp quadrants re/im
+//   hqp      [w][4][nxy]  dH_qp (H20 re, H02 re, H20 im, H02 im)
+//   dens     [w][2][56][nghl]   local densities of the two sectors
+//   flds     [w][2][208][nghl]  mean fields mf(z,z',sa,sb) and pairing fields
+// Converged frequencies leave the active list and are skipped by every launch.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/pnfam_gpu.h"
+#include "plan.hpp"
+
+using namespace pnfam;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                          \
+  do {                                                                                         \
+    hipError_t e_ = (expr);                                                                    \
+    if (e_ != hipSuccess)                                                                      \
+      return fail(PNFAM_GPU_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));       \
+  } while (0)
+
+constexpr int NDENS = 28;          // complex local densities per sector
+constexpr int NFLD = 104;          // complex fields per sector: 100 mf(z,z',sa,sb) + 4 pairing
+constexpr double PI = 3.14159265358979323846;
+
+// density indices (order of the density_set, pnfam_type_densityset.f90:14-33)
+enum { D_RHO, D_TAU, D_TJRR, D_TJPR, D_TJZR, D_TJRP, D_TJPP, D_TJZP, D_TJRZ, D_TJPZ, D_TJZZ,
+       D_SR, D_SP, D_SZ, D_TR, D_TP, D_TZ, D_JR, D_JP, D_JZ, D_FR, D_FP, D_FZ, D_GS,
+       D_RB, D_SBR, D_SBP, D_SBZ };
+
+__host__ __device__ constexpr int mf_index(int z, int zp, int sa, int sb) {  // sa/sb: 0 up, 1 down
+  return ((z * 5 + zp) * 2 + sa) * 2 + sb;
+}
+
+struct DevPtr {
+  void *p = nullptr;
+  size_t bytes = 0;
+  ~DevPtr() { if (p) (void)hipFree(p); }
+  int alloc(size_t n) {
+    if (p && bytes >= n) return 0;
+    if (p) { (void)hipFree(p); p = nullptr; bytes = 0; }
+    if (n == 0) n = 8;
+    if (hipMalloc(&p, n) != hipSuccess) { p = nullptr; return fail(PNFAM_GPU_ENOMEM, "hipMalloc of " + std::to_string(n) + " bytes failed"); }
+    bytes = n;
+    return 0;
+  }
+  template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+// ---------------------------------------------------------------------------
+// Kernels
+// ---------------------------------------------------------------------------
+
+__global__ void k_fill(double *__restrict__ p, size_t n, double v) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) p[i] = v;
+}
+
+// zero slots [s0, s0+ns) of every active frequency's [w][nslot][nxy] array
+__global__ void k_zero_slots(double *__restrict__ base, const int *__restrict__ act, int nslot, int s0, int ns,
+                             long nxy) {
+  int w = act[blockIdx.y];
+  double *p = base + ((size_t)w * nslot + s0) * nxy;
+  size_t n = (size_t)ns * nxy;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0.0;
+}
+
+// Block triple products out[i,j] = sum_terms s op(A) op(B_w) op(C); one workgroup per
+// (output block, frequency).  T = s op(A) op(B) is staged in LDS.
+__global__ void __launch_bounds__(256) k_triprod(const Group *__restrict__ groups, const Term *__restrict__ terms,
+                                                 const double *__restrict__ m0, const double *__restrict__ m1,
+                                                 const double *__restrict__ m2, const double *__restrict__ m3,
+                                                 const double *__restrict__ bbase, long bstride, int bslots,
+                                                 double *__restrict__ obase, long ostride, int oslots,
+                                                 const int *__restrict__ act, long nxy) {
+  extern __shared__ double lds[];
+  const Group g = groups[blockIdx.x];
+  const int w = act ? act[blockIdx.y] : (int)blockIdx.y;
+  const double *B = bbase + (size_t)w * bstride;
+  double *O = obase + (size_t)w * ostride + (size_t)g.out_slot * nxy + g.out_off;
+  const double *mats[4] = {m0, m1, m2, m3};
+  const int n = g.nrow * g.ncol;
+  for (int t = 0; t < g.count; ++t) {
+    const Term tm = terms[g.first + t];
+    const double *A = mats[tm.a_id] + tm.a_off;
+    const double *Bb = B + (size_t)tm.b_slot * nxy + tm.b_off;
+    const double *C = mats[tm.c_id] + tm.c_off;
+    const int nda = tm.nda, ndc = tm.ndc;
+    for (int e = threadIdx.x; e < n; e += blockDim.x) {
+      const int i = e % nda, j = e / nda;
+      double acc = 0.0;
+      for (int k = 0; k < nda; ++k) {
+        const double a = tm.ta ? A[k + i * nda] : A[i + k * nda];
+        const double b = tm.tb ? Bb[j + k * tm.ldb] : Bb[k + j * tm.ldb];
+        acc += a * b;
+      }
+      lds[e] = tm.s * acc;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < n; e += blockDim.x) {
+      const int i = e % nda, j = e / nda;
+      double acc = 0.0;
+      for (int l = 0; l < ndc; ++l) {
+        const double c = tm.tc ? C[j + l * ndc] : C[l + j * ndc];
+        acc += lds[i + l * nda] * c;
+      }
+      O[e] = (t == 0) ? acc : O[e] + acc;
+    }
+    __syncthreads();
+  }
+}
+
+struct PairList {       // block pairs (row block, col block, element offset) of a block map
+  const int4 *p;
+  int n;
+};
+
+struct DensArgs {
+  const double *wf, *wfdr, *wfdp, *wfdz, *wdcori;
+  const int *isstart, *db, *nsu, *ns;
+  const double *rsp;   // [w][8][nxy]
+  long nxy;
+  int nghl;
+  PairList rho[2], kap[2];   // per sector
+  int rho_slot[2], kap_slot[2];
+  double *dens;        // [w][2][56][nghl]
+  const int *act;
+};
+
+// Local densities (pnfam_hamiltonian_blas.f90:124-711): per grid point r and
+// column b, T_x^s(r,b) = sum_{a in block, spin s} psi_x(r,a) rho(a,b) followed by
+// the products with psi_y(r,b).  One lane per grid point; a workgroup loops over
+// all block pairs, so every density is a deterministic per-thread sum.
+__global__ void __launch_bounds__(64) k_density(DensArgs A) {
+  const int r = blockIdx.x * 64 + threadIdx.x;
+  const int w = A.act[blockIdx.y];
+  const int sec = blockIdx.z;
+  const int G = A.nghl;
+  const bool live = r < G;
+  const int rr = live ? r : 0;
+  double acc[2 * NDENS];
+#pragma unroll
+  for (int d = 0; d < 2 * NDENS; ++d) acc[d] = 0.0;
+  const double *rho_re = A.rsp + ((size_t)w * 8 + 2 * A.rho_slot[sec]) * A.nxy;
+  const double *rho_im = rho_re + A.nxy;
+  const double *tab[4] = {A.wf, A.wfdr, A.wfdp, A.wfdz};
+#define ACC(d, ri) acc[2 * (d) + (ri)]
+  const PairList pl = A.rho[sec];
+  for (int p = 0; p < pl.n; ++p) {
+    const int4 pr = pl.p[p];
+    const int ix = pr.x, iy = pr.y, off = pr.z;
+    const int ia = A.isstart[ix], ib0 = A.isstart[iy], dx = A.db[ix], dy = A.db[iy], nsu = A.nsu[ix];
+    for (int bl = 0; bl < dy; ++bl) {
+      const int b = ib0 + bl;
+      double Tr[4][2], Ti[4][2];  // [x][spin of a: 0 up, 1 down]
+#pragma unroll
+      for (int x = 0; x < 4; ++x) { Tr[x][0] = Tr[x][1] = Ti[x][0] = Ti[x][1] = 0.0; }
+      const double *cr = rho_re + off + (size_t)bl * dx;
+      const double *ci = rho_im + off + (size_t)bl * dx;
+      for (int al = 0; al < dx; ++al) {
+        const int sp = al < nsu ? 0 : 1;
+        const double vr = cr[al], vi = ci[al];
+        const size_t o = (size_t)(ia + al) * G + rr;
+#pragma unroll
+        for (int x = 0; x < 4; ++x) {
+          const double psi = tab[x][o];
+          Tr[x][sp] += psi * vr;
+          Ti[x][sp] += psi * vi;
+        }
+      }
+      const int s = A.ns[b];
+      const int sd = s > 0 ? 0 : 1, so = 1 - sd;
+      const size_t ob = (size_t)b * G + rr;
+      const double y[4] = {A.wf[ob], A.wfdr[ob], A.wfdp[ob], A.wfdz[ob]};
+      // P[x][y] = T_x(spin of b) psi_y(b); Q[x][y] = T_x(opposite spin) psi_y(b)
+      double Pr[4][4], Pi[4][4], Qr[4][4], Qi[4][4];
+#pragma unroll
+      for (int x = 0; x < 4; ++x)
+#pragma unroll
+        for (int yy = 0; yy < 4; ++yy) {
+          Pr[x][yy] = Tr[x][sd] * y[yy]; Pi[x][yy] = Ti[x][sd] * y[yy];
+          Qr[x][yy] = Tr[x][so] * y[yy]; Qi[x][yy] = Ti[x][so] * y[yy];
+        }
+      double ar, ai;
+      ACC(D_RHO, 0) += Pr[0][0]; ACC(D_RHO, 1) += Pi[0][0];
+      ACC(D_SZ, 0) += s * Pr[0][0]; ACC(D_SZ, 1) += s * Pi[0][0];
+      ar = 0.5 * (Pr[0][2] + Pr[2][0]); ai = 0.5 * (Pi[0][2] + Pi[2][0]);
+      ACC(D_JP, 0) += ar; ACC(D_JP, 1) += ai;
+      ACC(D_TJPZ, 0) += s * ar; ACC(D_TJPZ, 1) += s * ai;
+      ar = Pr[1][1] + Pr[2][2] + Pr[3][3]; ai = Pi[1][1] + Pi[2][2] + Pi[3][3];
+      ACC(D_TAU, 0) += ar; ACC(D_TAU, 1) += ai;
+      ACC(D_TZ, 0) += s * ar; ACC(D_TZ, 1) += s * ai;
+      ar = 0.5 * (Pr[0][1] - Pr[1][0]); ai = 0.5 * (Pi[0][1] - Pi[1][0]);   // j_r = i*aux
+      ACC(D_JR, 0) -= ai; ACC(D_JR, 1) += ar;
+      ACC(D_TJRZ, 0) -= s * ai; ACC(D_TJRZ, 1) += s * ar;
+      ar = 0.5 * (Pr[0][3] - Pr[3][0]); ai = 0.5 * (Pi[0][3] - Pi[3][0]);
+      ACC(D_JZ, 0) -= ai; ACC(D_JZ, 1) += ar;
+      ACC(D_TJZZ, 0) -= s * ai; ACC(D_TJZZ, 1) += s * ar;
+      ACC(D_GS, 0) += s * (Pr[0][3] + Pr[3][0]); ACC(D_GS, 1) += s * (Pi[0][3] + Pi[3][0]);
+      ACC(D_FR, 0) += 0.5 * s * (Pr[1][3] + Pr[3][1]); ACC(D_FR, 1) += 0.5 * s * (Pi[1][3] + Pi[3][1]);
+      ar = Pr[2][3] - Pr[3][2]; ai = Pi[2][3] - Pi[3][2];
+      ACC(D_FP, 0) -= 0.5 * s * ai; ACC(D_FP, 1) += 0.5 * s * ar;
+      ACC(D_FZ, 0) += s * Pr[3][3]; ACC(D_FZ, 1) += s * Pi[3][3];
+      // spin-flip part; e = -s: +1 for (a up, b down), -1 for (a down, b up)
+      const double e = -(double)s;
+      ACC(D_SR, 0) += Qr[0][0]; ACC(D_SR, 1) += Qi[0][0];
+      ACC(D_SP, 0) -= e * Qi[0][0]; ACC(D_SP, 1) += e * Qr[0][0];
+      const double t2r = 0.5 * (Qr[2][0] + Qr[0][2]), t2i = 0.5 * (Qi[2][0] + Qi[0][2]);
+      ACC(D_TJPR, 0) += t2r; ACC(D_TJPR, 1) += t2i;
+      ACC(D_TJPP, 0) -= e * t2i; ACC(D_TJPP, 1) += e * t2r;
+      ACC(D_GS, 0) += e * (Qr[0][2] - Qr[2][0]) + (Qr[0][1] + Qr[1][0]);
+      ACC(D_GS, 1) += e * (Qi[0][2] - Qi[2][0]) + (Qi[0][1] + Qi[1][0]);
+      const double tkr = Qr[1][1] + Qr[2][2] + Qr[3][3], tki = Qi[1][1] + Qi[2][2] + Qi[3][3];
+      ACC(D_TR, 0) += tkr; ACC(D_TR, 1) += tki;
+      ACC(D_TP, 0) -= e * tki; ACC(D_TP, 1) += e * tkr;
+      const double jrr = 0.5 * (Qr[0][1] - Qr[1][0]), jri = 0.5 * (Qi[0][1] - Qi[1][0]);
+      ACC(D_TJRR, 0) -= jri; ACC(D_TJRR, 1) += jrr;
+      ACC(D_TJRP, 0) -= e * jrr; ACC(D_TJRP, 1) -= e * jri;
+      const double jzr = 0.5 * (Qr[0][3] - Qr[3][0]), jzi = 0.5 * (Qi[0][3] - Qi[3][0]);
+      ACC(D_TJZR, 0) -= jzi; ACC(D_TJZR, 1) += jzr;
+      ACC(D_TJZP, 0) -= e * jzr; ACC(D_TJZP, 1) -= e * jzi;
+      ar = Qr[1][1] + e * 0.5 * (Qr[1][2] - Qr[2][1]);
+      ai = Qi[1][1] + e * 0.5 * (Qi[1][2] - Qi[2][1]);
+      ACC(D_FR, 0) += ar; ACC(D_FR, 1) += ai;
+      ar = e * Qr[2][2] + 0.5 * (Qr[2][1] - Qr[1][2]);
+      ai = e * Qi[2][2] + 0.5 * (Qi[2][1] - Qi[1][2]);
+      ACC(D_FP, 0) -= ai; ACC(D_FP, 1) += ar;
+      ar = Qr[3][1] + Qr[1][3] + e * (Qr[3][2] - Qr[2][3]);
+      ai = Qi[3][1] + Qi[1][3] + e * (Qi[3][2] - Qi[2][3]);
+      ACC(D_FZ, 0) += 0.5 * ar; ACC(D_FZ, 1) += 0.5 * ai;
+    }
+  }
+  // pairing densities from kappa (pnfam_hamiltonian_blas.f90:596-675)
+  const double *kap_re = A.rsp + ((size_t)w * 8 + 2 * A.kap_slot[sec]) * A.nxy;
+  const double *kap_im = kap_re + A.nxy;
+  const PairList kl = A.kap[sec];
+  for (int p = 0; p < kl.n; ++p) {
+    const int4 pr = kl.p[p];
+    const int ix = pr.x, iy = pr.y, off = pr.z;
+    const int ia = A.isstart[ix], ib0 = A.isstart[iy], dx = A.db[ix], dy = A.db[iy], nsu = A.nsu[ix];
+    for (int bl = 0; bl < dy; ++bl) {
+      const int b = ib0 + bl;
+      double Tr[2] = {0.0, 0.0}, Ti[2] = {0.0, 0.0};
+      const double *cr = kap_re + off + (size_t)bl * dx;
+      const double *ci = kap_im + off + (size_t)bl * dx;
+      for (int al = 0; al < dx; ++al) {
+        const int sp = al < nsu ? 0 : 1;
+        const double psi = A.wf[(size_t)(ia + al) * G + rr];
+        Tr[sp] += psi * cr[al];
+        Ti[sp] += psi * ci[al];
+      }
+      const int s = A.ns[b];
+      const int sd = s > 0 ? 0 : 1, so = 1 - sd;
+      const double wb = A.wf[(size_t)b * G + rr];
+      double pr_ = 2.0 * Tr[so] * wb, pi_ = 2.0 * Ti[so] * wb;
+      ACC(D_RB, 0) -= s * pr_; ACC(D_RB, 1) -= s * pi_;
+      ACC(D_SBZ, 0) += pr_; ACC(D_SBZ, 1) += pi_;
+      pr_ = 2.0 * Tr[sd] * wb; pi_ = 2.0 * Ti[sd] * wb;
+      ACC(D_SBR, 0) += s < 0 ? pr_ : -pr_; ACC(D_SBR, 1) += s < 0 ? pi_ : -pi_;
+      ACC(D_SBP, 0) += pi_; ACC(D_SBP, 1) -= pr_;
+    }
+  }
+#undef ACC
+  if (!live) return;
+  const double wc = A.wdcori[r];
+  double *out = A.dens + (((size_t)w * 2 + sec) * 2 * NDENS) * G + r;
+#pragma unroll
+  for (int d = 0; d < 2 * NDENS; ++d) out[(size_t)d * G] = acc[d] * wc;
+}
+
+struct FieldArgs {
+  const double *dens;     // [w][2][56][nghl]
+  double *flds;           // [w][2][208][nghl]
+  const double *crho, *cs, *cpair, *cspair;
+  double cdrho, ctau, ctj0, ctj1, ctj2, crdj, cds, ct, cj, cgs, cf, csdj;
+  int nghl;
+  const int *act;
+};
+
+struct cplx { double r, i; };
+__device__ inline cplx operator+(cplx a, cplx b) { return {a.r + b.r, a.i + b.i}; }
+__device__ inline cplx operator-(cplx a, cplx b) { return {a.r - b.r, a.i - b.i}; }
+__device__ inline cplx operator*(double s, cplx a) { return {s * a.r, s * a.i}; }
+__device__ inline cplx mul_i(cplx a) { return {-a.i, a.r}; }   // i*a
+
+// Mean and pairing fields in closed form.  The sequence of assignments of
+// pnfam_hamiltonian_blas.f90:774-1093 (and 1200-1208 for pairing) resolved per
+// field mf(z, z', s_a, s_b) with s = +1/-1 (index 0/1 here).
+__global__ void __launch_bounds__(128) k_fields(FieldArgs F) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= F.nghl) return;
+  const int w = F.act[blockIdx.y];
+  const int sec = blockIdx.z;
+  const int G = F.nghl;
+  const double *dn = F.dens + (((size_t)w * 2 + sec) * 2 * NDENS) * G + r;
+  auto D = [&](int d) { return cplx{dn[(size_t)(2 * d) * G], dn[(size_t)(2 * d + 1) * G]}; };
+  const cplx rho = D(D_RHO), tau = D(D_TAU), sr = D(D_SR), sp = D(D_SP), sz = D(D_SZ), tr = D(D_TR), tp = D(D_TP),
+             tz = D(D_TZ), jr = D(D_JR), jp = D(D_JP), jz = D(D_JZ), fr = D(D_FR), fp = D(D_FP), fz = D(D_FZ),
+             gs = D(D_GS), tjrr = D(D_TJRR), tjpr = D(D_TJPR), tjzr = D(D_TJZR), tjrp = D(D_TJRP),
+             tjpp = D(D_TJPP), tjzp = D(D_TJZP), tjrz = D(D_TJRZ), tjpz = D(D_TJPZ), tjzz = D(D_TJZZ),
+             rb = D(D_RB), sbr = D(D_SBR), sbp = D(D_SBP), sbz = D(D_SBZ);
+  const double crho = F.crho[r], cs = F.cs[r], cpair = F.cpair[r], cspair = F.cspair[r];
+  const double cdrho = F.cdrho, ctau = F.ctau, ctj0 = F.ctj0, ctj1 = F.ctj1, ctj2 = F.ctj2, crdj = F.crdj,
+               cds = F.cds, ct = F.ct, cj = F.cj, cgs = F.cgs, cf = F.cf, csdj = F.csdj;
+  const cplx c0 = ctj0 * (tjrr + tjpp + tjzz), c1a = ctj1 * (tjzr - tjrz), c1b = ctj1 * (tjpz - tjzp),
+             c2a = ctj2 * (tjrz + tjzr), c2b = ctj2 * (tjpz + tjzp);
+  double *out = F.flds + (((size_t)w * 2 + sec) * 2 * NFLD) * G + r;
+  auto put = [&](int idx, cplx v) { out[(size_t)(2 * idx) * G] = v.r; out[(size_t)(2 * idx + 1) * G] = v.i; };
+  for (int k = 0; k < NFLD; ++k) put(k, cplx{0.0, 0.0});
+  // spin-diagonal fields, sigma = +1 (index 0) and -1 (index 1)
+  for (int si = 0; si < 2; ++si) {
+    const double sg = si == 0 ? 1.0 : -1.0;
+    put(mf_index(0, 0, si, si), 2.0 * crho * rho + ctau * tau + sg * (2.0 * cs * sz + ct * tz + cf * fz));
+    const cplx b01 = -crdj * (tjpz - tjzp) - sg * csdj * jp;
+    const cplx x01 = sg * mul_i(c1a - 0.5 * c2a) - mul_i(cj * jr);
+    put(mf_index(0, 1, si, si), b01 + x01);
+    put(mf_index(1, 0, si, si), b01 - x01);
+    const cplx b02 = cj * jp + sg * (c1b + 0.5 * c2b);
+    const cplx x02 = sg * mul_i(csdj * jr) - mul_i(crdj * (tjzr - tjrz));
+    put(mf_index(0, 2, si, si), b02 + x02);
+    put(mf_index(2, 0, si, si), b02 - x02);
+    const cplx b03 = -crdj * (tjrp - tjpr) + 2.0 * sg * cgs * gs;
+    const cplx x03 = -sg * mul_i(c0 - (ctj2 / 3.0) * (tjrr + tjpp - 2.0 * tjzz)) - mul_i(cj * jz);
+    put(mf_index(0, 3, si, si), b03 + x03);
+    put(mf_index(3, 0, si, si), b03 - x03);
+    const cplx f04 = 2.0 * cdrho * rho + 2.0 * sg * cds * sz;
+    put(mf_index(0, 4, si, si), f04);
+    put(mf_index(4, 0, si, si), f04);
+    const cplx f12 = csdj * sz + sg * crdj * rho;
+    put(mf_index(1, 2, si, si), f12);
+    put(mf_index(2, 1, si, si), f12);
+    put(mf_index(1, 3, si, si), sg * 0.5 * cf * sr + mul_i(csdj * sp));
+    put(mf_index(3, 1, si, si), sg * 0.5 * cf * sr - mul_i(csdj * sp));
+    put(mf_index(2, 3, si, si), -csdj * sr - sg * mul_i(0.5 * cf * sp));
+    put(mf_index(3, 2, si, si), -csdj * sr + sg * mul_i(0.5 * cf * sp));
+    const cplx f11 = (4.0 * cdrho + ctau) * rho + sg * (4.0 * cds + ct) * sz;
+    put(mf_index(1, 1, si, si), f11);
+    put(mf_index(2, 2, si, si), f11);
+    put(mf_index(3, 3, si, si), f11 + sg * cf * sz);
+  }
+  // spin-flip fields: (sa, sb) = (up, down) eps = +1, (down, up) eps = -1
+  const cplx a2 = -ctj1 * (tjrp - tjpr);
+  const cplx a3 = -1.0 * mul_i(c0);
+  const cplx a4 = -0.5 * ctj2 * (tjrp + tjpr);
+  const cplx a5 = mul_i((ctj2 / 3.0) * (-2.0 * tjrr + tjpp + tjzz));
+  const cplx a6 = mul_i((ctj2 / 3.0) * (tjrr - 2.0 * tjpp + tjzz));
+  for (int k = 0; k < 2; ++k) {
+    const int sa = k, sb = 1 - k;
+    const double e = k == 0 ? 1.0 : -1.0;
+    put(mf_index(0, 0, sa, sb), (2.0 * cs * sr + ct * tr + cf * fr) - e * mul_i(2.0 * cs * sp + ct * tp + cf * fp));
+    const cplx g = 2.0 * cgs * gs;
+    const cplx iz = mul_i(csdj * jz);
+    put(mf_index(0, 1, sa, sb), g - e * iz + e * a2 + a3 + e * a4 + a5);
+    put(mf_index(1, 0, sa, sb), g - e * iz - e * a2 - a3 - e * a4 - a5);
+    put(mf_index(0, 2, sa, sb), e * g - iz + a2 + e * a3 - a4 + e * a6);
+    put(mf_index(2, 0, sa, sb), -e * g + iz + a2 + e * a3 - a4 + e * a6);
+    const cplx b3 = csdj * jp + e * mul_i(csdj * jr);
+    put(mf_index(0, 3, sa, sb), b3 - mul_i(c1a + 0.5 * c2a) + e * (c1b - 0.5 * c2b));
+    put(mf_index(3, 0, sa, sb), b3 + mul_i(c1a + 0.5 * c2a) - e * (c1b - 0.5 * c2b));
+    const cplx f04 = 2.0 * cds * (sr - e * mul_i(sp));
+    put(mf_index(0, 4, sa, sb), f04);
+    put(mf_index(4, 0, sa, sb), f04);
+    const cplx f12 = mul_i(0.5 * cf * sp) + e * 0.5 * cf * sr;
+    put(mf_index(1, 2, sa, sb), f12);
+    put(mf_index(2, 1, sa, sb), -1.0 * f12);
+    put(mf_index(1, 3, sa, sb), 0.5 * cf * sz + e * crdj * rho);
+    put(mf_index(3, 1, sa, sb), 0.5 * cf * sz - e * crdj * rho);
+    put(mf_index(2, 3, sa, sb), -e * 0.5 * cf * sz - crdj * rho);
+    put(mf_index(3, 2, sa, sb), e * 0.5 * cf * sz - crdj * rho);
+    const cplx f33 = (ct + 4.0 * cds) * (sr - e * mul_i(sp));
+    put(mf_index(3, 3, sa, sb), f33);
+    put(mf_index(1, 1, sa, sb), f33 + cf * sr);
+    put(mf_index(2, 2, sa, sb), f33 - e * mul_i(cf * sp));
+  }
+  // pairing fields P(sa, sb) (pnfam_hamiltonian_blas.f90:1200-1208)
+  put(100 + 0, cspair * (sbr - mul_i(sbp)));             // (up, up)
+  put(100 + 1, -cpair * rb - cspair * sbz);              // (down, up)
+  put(100 + 2, cpair * rb - cspair * sbz);               // (up, down)
+  put(100 + 3, cspair * (-1.0 * sbr - mul_i(sbp)));      // (down, down)
+}
+
+struct Tile {        // output tile of one block of a projected matrix
+  int32_t ix, iy, off, sec;   // row block, col block, element offset, sector
+  int32_t a0, b0, na, nb_;    // tile origin / size inside the block
+  int32_t out_slot, kind, pad0, pad1;   // kind 0: mean field (5 psi), 1: pairing (psi only)
+};
+static_assert(sizeof(Tile) == 48, "Tile layout");
+
+struct ProjArgs {
+  const double *psi[5];   // wf, wfdr, wfdp, wfdz, wfd2_all
+  const int *isstart, *db, *nsu, *ns;
+  const double *flds;     // [w][2][208][nghl]
+  double *hsp;            // [w][8][nxy]
+  long nxy;
+  int nghl;
+  const Tile *tiles;
+  const int *act;
+};
+
+constexpr int PT = 32;   // tile edge
+constexpr int PR = 8;    // grid points per LDS chunk
+
+// h_ab = 2 sum_r sum_z psi_z(r,a) hpsi_z^{s_a}(r,b),
+// hpsi_z^{s_a}(r,b) = sum_z' mf(z,z',s_a,s_b)(r) psi_z'(r,b)      (:1105-1167)
+// Delta_ab = 2 sum_r psi_0(r,a) P_{s_a s_b}(r) psi_0(r,b)        (:1210-1260)
+__global__ void __launch_bounds__(256) k_project(ProjArgs P) {
+  __shared__ double sA[5][PR][PT];
+  __shared__ double sH[5][2][2][PR][PT];  // [z][s_a][re/im][r][b]
+  const Tile t = P.tiles[blockIdx.x];
+  const int w = P.act[blockIdx.y];
+  const int G = P.nghl;
+  const int ia = P.isstart[t.ix] + t.a0, ib = P.isstart[t.iy] + t.b0;
+  const int nsu = P.nsu[t.ix];
+  const int nz = t.kind == 0 ? 5 : 1;
+  const double *fl = P.flds + (((size_t)w * 2 + t.sec) * 2 * NFLD) * G;
+  // each thread owns 4 output elements: (a = tid%32, b = tid/32 + 8k)
+  const int ta = threadIdx.x % PT, tb0 = threadIdx.x / PT;
+  double accr[4] = {0, 0, 0, 0}, acci[4] = {0, 0, 0, 0};
+  const int sa_own = (t.a0 + ta) < nsu ? 0 : 1;
+  for (int r0 = 0; r0 < G; r0 += PR) {
+    // stage psi_z(r, a)
+    for (int e = threadIdx.x; e < nz * PR * PT; e += 256) {
+      const int a = e % PT, rr = (e / PT) % PR, z = e / (PT * PR);
+      const int r = r0 + rr;
+      sA[z][rr][a] = (a < t.na && r < G) ? P.psi[z][(size_t)(ia + a) * G + r] : 0.0;
+    }
+    // hpsi for (rr, b): one pair per thread
+    {
+      const int b = threadIdx.x % PT, rr = threadIdx.x / PT;
+      const int r = r0 + rr;
+      if (b < t.nb_ && r < G) {
+        const size_t ob = (size_t)(ib + b) * G + r;
+        const int sb = P.ns[ib + b] > 0 ? 0 : 1;
+        if (t.kind == 0) {
+          double pv[5];
+#pragma unroll
+          for (int z = 0; z < 5; ++z) pv[z] = P.psi[z][ob];
+#pragma unroll
+          for (int sa = 0; sa < 2; ++sa)
+#pragma unroll
+            for (int z = 0; z < 5; ++z) {
+              double hr = 0.0, hi = 0.0;
+#pragma unroll
+              for (int zp = 0; zp < 5; ++zp) {
+                const int idx = mf_index(z, zp, sa, sb);
+                hr += fl[(size_t)(2 * idx) * G + r] * pv[zp];
+                hi += fl[(size_t)(2 * idx + 1) * G + r] * pv[zp];
+              }
+              sH[z][sa][0][rr][b] = hr;
+              sH[z][sa][1][rr][b] = hi;
+            }
+        } else {
+          const double p0 = P.psi[0][ob];
+#pragma unroll
+          for (int sa = 0; sa < 2; ++sa) {
+            const int idx = 100 + (sb == 0 ? (sa == 0 ? 0 : 1) : (sa == 0 ? 2 : 3));
+            sH[0][sa][0][rr][b] = fl[(size_t)(2 * idx) * G + r] * p0;
+            sH[0][sa][1][rr][b] = fl[(size_t)(2 * idx + 1) * G + r] * p0;
+          }
+        }
+      } else {
+        for (int z = 0; z < nz; ++z)
+          for (int sa = 0; sa < 2; ++sa) { sH[z][sa][0][rr][b] = 0.0; sH[z][sa][1][rr][b] = 0.0; }
+      }
+    }
+    __syncthreads();
+    for (int k = 0; k < 4; ++k) {
+      const int b = tb0 + 8 * k;
+      double hr = 0.0, hi = 0.0;
+      for (int z = 0; z < nz; ++z)
+#pragma unroll
+        for (int rr = 0; rr < PR; ++rr) {
+          const double a = sA[z][rr][ta];
+          hr += a * sH[z][sa_own][0][rr][b];
+          hi += a * sH[z][sa_own][1][rr][b];
+        }
+      accr[k] += hr;
+      acci[k] += hi;
+    }
+    __syncthreads();
+  }
+  const int dx = P.db[t.ix];
+  double *ore = P.hsp + ((size_t)w * 8 + 2 * t.out_slot) * P.nxy + t.off;
+  double *oim = ore + P.nxy;
+  for (int k = 0; k < 4; ++k) {
+    const int b = tb0 + 8 * k;
+    if (ta < t.na && b < t.nb_) {
+      const size_t o = (size_t)(t.a0 + ta) + (size_t)(t.b0 + b) * dx;
+      ore[o] = 2.0 * accr[k];
+      oim[o] = 2.0 * acci[k];
+    }
+  }
+}
+
+struct GreenArgs {
+  const double *hqp;     // [w][4][nxy]
+  const double *f20, *f02;
+  const double *e1, *e2; // row / column qp energies (Ep,En for beta-)
+  const int *xrow, *xcol, *yrow, *ycol;
+  const double *wre, *wim;   // per frequency
+  double *vout;          // [w][4][nxy]
+  long nxy;
+  double quench;
+  int use_h;
+  const int *act;
+};
+
+// X = g_X (F20 + q H20), Y = g_Y (F02 + q H02), g = -1/(E1_i + E2_j -+ w)
+// (pnfam_solver.f90:169-179, matrix_2qp :510-544)
+__global__ void k_green(GreenArgs A) {
+  const int w = A.act[blockIdx.y];
+  const long n = A.nxy;
+  const double wr = A.wre[w], wi = A.wim[w];
+  const double *h = A.hqp + (size_t)w * 4 * n;
+  double *v = A.vout + (size_t)w * 4 * n;
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < 2 * n; e += (long)gridDim.x * blockDim.x) {
+    const bool isx = e < n;
+    const long k = isx ? e : e - n;
+    // quench the induced field, then add the (real) external field (:169-175)
+    const double f = isx ? A.f20[k] : A.f02[k];
+    const double hr = A.use_h ? A.quench * h[(isx ? 0 : n) + k] + f : f;
+    const double hi = A.use_h ? A.quench * h[(isx ? 2 * n : 3 * n) + k] : 0.0;
+    const double E = isx ? A.e1[A.xrow[k]] + A.e2[A.xcol[k]] : A.e1[A.yrow[k]] + A.e2[A.ycol[k]];
+    // x = E -+ w ; g = -1/x
+    const double xr = isx ? E - wr : E + wr;
+    const double xi = isx ? -wi : wi;
+    const double den = xr * xr + xi * xi;
+    const double gr = -xr / den, gi = xi / den;
+    v[(isx ? 0 : n) + k] = gr * hr - gi * hi;
+    v[(isx ? 2 * n : 3 * n) + k] = gi * hr + hi * gr;
+  }
+}
+
+// ---- Broyden (pnfam_broyden.f90:117-216), per frequency, FP64 ---------------
+constexpr int BRO_WG = 256;
+constexpr int BRO_MAXM = 64;
+
+struct BroArgs {
+  double *vin, *vout;     // [w][N]
+  double *df, *dv;        // [w][M][N]
+  double *part;           // [w][nwg][2*M+2]
+  double *gram;           // [w][M][M]
+  double *coef;           // [w][M] gamma
+  double *si;             // [w]
+  double *scal;           // [w][4]: normi, ...
+  long N;
+  int M, nwg, iter, ipos, inext, iter_used;
+  double alpha;
+  const int *act;
+};
+
+__device__ inline double wg_sum(double v, double *red) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  const int lane = threadIdx.x % 64, wid = threadIdx.x / 64;
+  __syncthreads();
+  if (lane == 0) red[wid] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0)
+    for (int k = 0; k < (int)(blockDim.x / 64); ++k) s += red[k];
+  return s;  // valid on thread 0
+}
+__device__ inline double wg_max(double v, double *red) {
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_down(v, o, 64));
+  const int lane = threadIdx.x % 64, wid = threadIdx.x / 64;
+  __syncthreads();
+  if (lane == 0) red[wid] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0)
+    for (int k = 0; k < (int)(blockDim.x / 64); ++k) s = fmax(s, red[k]);
+  return s;
+}
+
+// vout <- vout - vin; partial max|vout|; for iter >= 2 also finalise the history
+// difference at slot ipos and its partial squared norm.
+__global__ void __launch_bounds__(BRO_WG) k_bro_resid(BroArgs B) {
+  __shared__ double red[BRO_WG / 64];
+  const int w = B.act[blockIdx.y];
+  const long N = B.N;
+  double *vo = B.vout + (size_t)w * N;
+  const double *vi = B.vin + (size_t)w * N;
+  const bool hist = B.iter >= 2 && B.M > 0;
+  double *dfp = B.df + ((size_t)w * B.M + (hist ? B.ipos : 0)) * N;
+  double *dvp = B.dv + ((size_t)w * B.M + (hist ? B.ipos : 0)) * N;
+  double mx = 0.0, nr = 0.0;
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < N; e += (long)gridDim.x * blockDim.x) {
+    const double d = vo[e] - vi[e];
+    vo[e] = d;
+    mx = fmax(mx, fabs(d));
+    if (hist) {
+      const double f = d - dfp[e];
+      dfp[e] = f;
+      dvp[e] = vi[e] - dvp[e];
+      nr += f * f;
+    }
+  }
+  const double m = wg_max(mx, red);
+  const double s = wg_sum(nr, red);
+  if (threadIdx.x == 0) {
+    double *pp = B.part + ((size_t)w * B.nwg + blockIdx.x) * (2 * BRO_MAXM + 2);
+    pp[0] = m;
+    pp[1] = s;
+  }
+}
+
+__global__ void k_bro_norm(BroArgs B) {
+  const int w = B.act[blockIdx.x];
+  if (threadIdx.x != 0) return;
+  double m = 0.0, s = 0.0;
+  for (int k = 0; k < B.nwg; ++k) {
+    const double *pp = B.part + ((size_t)w * B.nwg + k) * (2 * BRO_MAXM + 2);
+    m = fmax(m, pp[0]);
+    s += pp[1];
+  }
+  B.si[w] = m;
+  B.scal[(size_t)w * 4 + 0] = (B.iter >= 2 && B.M > 0) ? 1.0 / sqrt(s) : 1.0;
+}
+
+// scale the history slot, then partial dots df_j . df_ipos and df_j . vout
+__global__ void __launch_bounds__(BRO_WG) k_bro_dots(BroArgs B) {
+  __shared__ double red[BRO_WG / 64];
+  const int w = B.act[blockIdx.y];
+  const long N = B.N;
+  const double normi = B.scal[(size_t)w * 4 + 0];
+  double *dfp = B.df + ((size_t)w * B.M + B.ipos) * N;
+  double *dvp = B.dv + ((size_t)w * B.M + B.ipos) * N;
+  const double *vo = B.vout + (size_t)w * N;
+  const long e0 = (long)blockIdx.x * blockDim.x + threadIdx.x, es = (long)gridDim.x * blockDim.x;
+  if (B.iter >= 2)
+    for (long e = e0; e < N; e += es) { dfp[e] *= normi; dvp[e] *= normi; }
+  double *pp = B.part + ((size_t)w * B.nwg + blockIdx.x) * (2 * BRO_MAXM + 2) + 2;
+  for (int j = 0; j < B.iter_used; ++j) {
+    const double *dfj = B.df + ((size_t)w * B.M + j) * N;
+    double g = 0.0, v = 0.0;
+    for (long e = e0; e < N; e += es) {
+      const double a = dfj[e];
+      g += a * dfp[e];
+      v += a * vo[e];
+    }
+    const double sg = wg_sum(g, red);
+    const double sv = wg_sum(v, red);
+    if (threadIdx.x == 0) { pp[2 * j] = sg; pp[2 * j + 1] = sv; }
+  }
+}
+
+// Gram row update, beta = (w0^2+1) I + offdiag(Gram), gamma = beta^-1 work
+// (Gauss-Jordan on one workgroup per frequency)
+__global__ void __launch_bounds__(256) k_bro_solve(BroArgs B) {
+  __shared__ double a[BRO_MAXM][2 * BRO_MAXM + 1];
+  __shared__ double work[BRO_MAXM];
+  __shared__ double fac[BRO_MAXM];
+  const int w = B.act[blockIdx.x];
+  const int M = B.M, k = B.iter_used;
+  double *gram = B.gram + (size_t)w * M * M;
+  // reduce partial dots
+  for (int j = threadIdx.x; j < k; j += blockDim.x) {
+    double sg = 0.0, sv = 0.0;
+    for (int q = 0; q < B.nwg; ++q) {
+      const double *pp = B.part + ((size_t)w * B.nwg + q) * (2 * BRO_MAXM + 2) + 2;
+      sg += pp[2 * j];
+      sv += pp[2 * j + 1];
+    }
+    if (B.iter >= 2) { gram[(size_t)B.ipos * M + j] = sg; gram[(size_t)j * M + B.ipos] = sg; }
+    work[j] = sv;
+  }
+  __syncthreads();
+  const double w0 = 0.010;
+  for (int e = threadIdx.x; e < k * 2 * k; e += blockDim.x) {
+    const int i = e / (2 * k), j = e % (2 * k);
+    double v;
+    if (j < k) v = (i == j) ? w0 * w0 + 1.0 : gram[(size_t)i * M + j];
+    else v = (j - k == i) ? 1.0 : 0.0;
+    a[i][j] = v;
+  }
+  __syncthreads();
+  __shared__ int piv_s;
+  for (int c = 0; c < k; ++c) {
+    if (threadIdx.x == 0) {
+      int piv = c;
+      for (int i = c + 1; i < k; ++i)
+        if (fabs(a[i][c]) > fabs(a[piv][c])) piv = i;
+      piv_s = piv;
+    }
+    __syncthreads();
+    const int piv = piv_s;
+    if (piv != c)
+      for (int j = threadIdx.x; j < 2 * k; j += blockDim.x) { double t = a[c][j]; a[c][j] = a[piv][j]; a[piv][j] = t; }
+    __syncthreads();
+    const double d = 1.0 / a[c][c];
+    __syncthreads();
+    for (int j = threadIdx.x; j < 2 * k; j += blockDim.x) a[c][j] *= d;
+    __syncthreads();
+    for (int i = threadIdx.x; i < k; i += blockDim.x) fac[i] = a[i][c];
+    __syncthreads();
+    for (int e = threadIdx.x; e < k * 2 * k; e += blockDim.x) {
+      const int i = e / (2 * k), j = e % (2 * k);
+      if (i != c) a[i][j] -= fac[i] * a[c][j];
+    }
+    __syncthreads();
+  }
+  for (int i = threadIdx.x; i < k; i += blockDim.x) {
+    double g = 0.0;
+    for (int j = 0; j < k; ++j) g += a[j][k + i] * work[j];   // beta(j,i)
+    B.coef[(size_t)w * M + i] = g;
+  }
+}
+
+// curv = alpha vout - sum_i gamma_i (dv_i + alpha df_i); df[inext] = vout;
+// dv[inext] = vin; vin += curv.  With iter_used = 0 this is also the first
+// Broyden step; mode 1/2 give linear mixing / no mixing.
+__global__ void __launch_bounds__(BRO_WG) k_bro_update(BroArgs B, int mode) {
+  const int w = B.act[blockIdx.y];
+  const long N = B.N;
+  double *vi = B.vin + (size_t)w * N;
+  const double *vo = B.vout + (size_t)w * N;
+  const double *gm = B.coef + (size_t)w * (B.M > 0 ? B.M : 1);
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < N; e += (long)gridDim.x * blockDim.x) {
+    const double r = vo[e];
+    if (mode == 2) { vi[e] += r; continue; }
+    if (mode == 1) { vi[e] += B.alpha * r; continue; }
+    double c = B.alpha * r;
+    for (int i = 0; i < B.iter_used; ++i) {
+      const size_t o = ((size_t)w * B.M + i) * N + e;
+      c -= gm[i] * (B.dv[o] + B.alpha * B.df[o]);
+    }
+    const size_t on = ((size_t)w * B.M + B.inext) * N + e;
+    B.df[on] = r;
+    B.dv[on] = vi[e];
+    vi[e] += c;
+  }
+}
+
+// S = -(1/pi) (F20.X + F02.Y) for re and im (contract_bbm, pnfam_type_bbm.f90:393-421)
+__global__ void __launch_bounds__(256) k_contract(const double *__restrict__ vin, const double *__restrict__ f20,
+                                                  const double *__restrict__ f02, long nxy, double *__restrict__ part,
+                                                  int nwg, const int *__restrict__ act) {
+  __shared__ double red[4];
+  const int w = act[blockIdx.y];
+  const double *v = vin + (size_t)w * 4 * nxy;
+  double sr = 0.0, si = 0.0;
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < nxy; e += (long)gridDim.x * blockDim.x) {
+    sr += f20[e] * v[e] + f02[e] * v[nxy + e];
+    si += f20[e] * v[2 * nxy + e] + f02[e] * v[3 * nxy + e];
+  }
+  const double a = wg_sum(sr, red);
+  const double b = wg_sum(si, red);
+  if (threadIdx.x == 0) { part[((size_t)w * nwg + blockIdx.x) * 2] = a; part[((size_t)w * nwg + blockIdx.x) * 2 + 1] = b; }
+}
+
+__global__ void k_contract_fin(const double *__restrict__ part, int nwg, double *__restrict__ S,
+                               const int *__restrict__ act) {
+  const int w = act[blockIdx.x];
+  if (threadIdx.x != 0) return;
+  double a = 0.0, b = 0.0;
+  for (int k = 0; k < nwg; ++k) { a += part[((size_t)w * nwg + k) * 2]; b += part[((size_t)w * nwg + k) * 2 + 1]; }
+  S[2 * w] = -a / PI;
+  S[2 * w + 1] = -b / PI;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Context
+// ---------------------------------------------------------------------------
+struct pnfam_gpu_ctx {
+  int nb = 0, dqp = 0, nghl = 0;
+  std::vector<int> db, isstart, nsu, ns;
+  std::vector<double> cscal;
+  hipStream_t stream = nullptr;
+  // resident
+  DevPtr d_wf, d_wfdr, d_wfdz, d_wfdp, d_wfd2, d_wdcori, d_Ep, d_En;
+  DevPtr d_U[4];
+  DevPtr d_crho, d_cs, d_cpair, d_cspair;
+  DevPtr d_isstart, d_db, d_nsu, d_ns;
+  // operator
+  bool have_op = false;
+  bool bminus = true;
+  long nxy = 0;
+  BlockMap fmap, xmap, ymap, rspmap[4], hspmap[4];
+  DevPtr d_f20, d_f02, d_xrow, d_xcol, d_yrow, d_ycol;
+  DevPtr d_terms_q2s[2], d_groups_q2s[2], d_terms_s2q[2], d_groups_s2q[2];
+  int ngroups_q2s[2] = {0, 0}, ngroups_s2q[2] = {0, 0};
+  size_t lds_q2s = 0, lds_s2q = 0;
+  DevPtr d_pairs;   // rho0, kap0, rho1, kap1 pair lists
+  int pair_off[4] = {0, 0, 0, 0}, pair_n[4] = {0, 0, 0, 0};
+  DevPtr d_tiles;
+  int ntiles = 0;
+  // per-frequency workspace
+  int cap = 0;
+  int hist_cap = -1;
+  DevPtr d_vin, d_vout, d_rsp, d_hsp, d_hqp, d_dens, d_flds, d_part, d_cpart, d_S, d_si, d_scal;
+  DevPtr d_df, d_dv, d_gram, d_coef, d_wre, d_wim, d_act;
+  int bro_nwg = 0;
+  ~pnfam_gpu_ctx() { if (stream) (void)hipStreamDestroy(stream); }
+};
+
+namespace {
+
+int upload(DevPtr &d, const void *h, size_t bytes, hipStream_t s) {
+  if (int e = d.alloc(bytes)) return e;
+  HIP_TRY(hipMemcpyAsync(d.p, h, bytes, hipMemcpyHostToDevice, s));
+  return 0;
+}
+
+std::vector<int4> pair_list(const BlockMap &m) {
+  std::vector<int4> v;
+  for (size_t i = 0; i < m.ir2c.size(); ++i)
+    if (m.ir2c[i] >= 0) v.push_back(make_int4((int)i, m.ir2c[i], m.ir2m[i], 0));
+  return v;
+}
+
+int ensure_workspace(pnfam_gpu_ctx *h, int npts, int M) {
+  if (npts <= h->cap && M == h->hist_cap) return 0;
+  const size_t n = (size_t)std::max(npts, 1);
+  const size_t nxy = (size_t)h->nxy;
+  const int Mh = std::max(M, 1);
+  int e;
+  if ((e = h->d_vin.alloc(n * 4 * nxy * 8))) return e;
+  if ((e = h->d_vout.alloc(n * 4 * nxy * 8))) return e;
+  if ((e = h->d_rsp.alloc(n * 8 * nxy * 8))) return e;
+  if ((e = h->d_hsp.alloc(n * 8 * nxy * 8))) return e;
+  if ((e = h->d_hqp.alloc(n * 4 * nxy * 8))) return e;
+  if ((e = h->d_dens.alloc(n * 2 * 2 * NDENS * (size_t)h->nghl * 8))) return e;
+  if ((e = h->d_flds.alloc(n * 2 * 2 * NFLD * (size_t)h->nghl * 8))) return e;
+  h->bro_nwg = (int)std::min<long>(64, std::max<long>(1, (long)(4 * nxy + BRO_WG - 1) / BRO_WG));
+  if ((e = h->d_part.alloc(n * h->bro_nwg * (2 * BRO_MAXM + 2) * 8))) return e;
+  if ((e = h->d_cpart.alloc(n * 64 * 2 * 8))) return e;
+  if ((e = h->d_S.alloc(n * 2 * 8))) return e;
+  if ((e = h->d_si.alloc(n * 8))) return e;
+  if ((e = h->d_scal.alloc(n * 4 * 8))) return e;
+  if ((e = h->d_df.alloc(n * Mh * 4 * nxy * 8))) return e;
+  if ((e = h->d_dv.alloc(n * Mh * 4 * nxy * 8))) return e;
+  if ((e = h->d_gram.alloc(n * Mh * Mh * 8))) return e;
+  if ((e = h->d_coef.alloc(n * Mh * 8))) return e;
+  if ((e = h->d_wre.alloc(n * 8))) return e;
+  if ((e = h->d_wim.alloc(n * 8))) return e;
+  if ((e = h->d_act.alloc(n * 4))) return e;
+  h->cap = (int)n;
+  h->hist_cap = M;
+  return 0;
+}
+
+dim3 grid1(size_t n, int bs, int maxb = 2048) {
+  size_t b = (n + bs - 1) / bs;
+  return dim3((unsigned)std::max<size_t>(1, std::min<size_t>(b, maxb)));
+}
+
+// ---- pipeline stages over the active list (device pointer act, count na) --
+int stage_qp2sp(pnfam_gpu_ctx *h, const double *vin, int na, const int *act) {
+  const long nxy = h->nxy;
+  hipLaunchKernelGGL(k_zero_slots, dim3(64, na), dim3(256), 0, h->stream, h->d_rsp.as<double>(), act, 8, 0, 8, nxy);
+  for (int ri = 0; ri < 2; ++ri) {
+    if (!h->ngroups_q2s[ri]) continue;
+    hipLaunchKernelGGL(k_triprod, dim3(h->ngroups_q2s[ri], na), dim3(256), h->lds_q2s, h->stream,
+                       h->d_groups_q2s[ri].as<Group>(), h->d_terms_q2s[ri].as<Term>(), h->d_U[0].as<double>(),
+                       h->d_U[1].as<double>(), h->d_U[2].as<double>(), h->d_U[3].as<double>(), vin, 4 * nxy, 4,
+                       h->d_rsp.as<double>(), 8 * nxy, 8, act, nxy);
+  }
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+int stage_hamiltonian(pnfam_gpu_ctx *h, int na, const int *act) {
+  DensArgs A{};
+  A.wf = h->d_wf.as<double>(); A.wfdr = h->d_wfdr.as<double>(); A.wfdp = h->d_wfdp.as<double>();
+  A.wfdz = h->d_wfdz.as<double>(); A.wdcori = h->d_wdcori.as<double>();
+  A.isstart = h->d_isstart.as<int>(); A.db = h->d_db.as<int>(); A.nsu = h->d_nsu.as<int>(); A.ns = h->d_ns.as<int>();
+  A.rsp = h->d_rsp.as<double>(); A.nxy = h->nxy; A.nghl = h->nghl;
+  const int4 *pl = h->d_pairs.as<int4>();
+  A.rho[0] = PairList{pl + h->pair_off[0], h->pair_n[0]};
+  A.kap[0] = PairList{pl + h->pair_off[1], h->pair_n[1]};
+  A.rho[1] = PairList{pl + h->pair_off[2], h->pair_n[2]};
+  A.kap[1] = PairList{pl + h->pair_off[3], h->pair_n[3]};
+  // sector 0: rho_pn = m11 (slot 0), kappa+ = m12 (slot 1); sector 1: rho_np = m22 (3), kappa- = m21 (2)
+  A.rho_slot[0] = 0; A.kap_slot[0] = 1; A.rho_slot[1] = 3; A.kap_slot[1] = 2;
+  A.dens = h->d_dens.as<double>(); A.act = act;
+  hipLaunchKernelGGL(k_density, dim3((h->nghl + 63) / 64, na, 2), dim3(64), 0, h->stream, A);
+  FieldArgs F{};
+  F.dens = h->d_dens.as<double>(); F.flds = h->d_flds.as<double>();
+  F.crho = h->d_crho.as<double>(); F.cs = h->d_cs.as<double>(); F.cpair = h->d_cpair.as<double>();
+  F.cspair = h->d_cspair.as<double>();
+  const double *c = h->cscal.data();
+  F.cdrho = c[0]; F.ctau = c[1]; F.ctj0 = c[2]; F.ctj1 = c[3]; F.ctj2 = c[4]; F.crdj = c[5];
+  F.cds = c[6]; F.ct = c[7]; F.cj = c[8]; F.cgs = c[9]; F.cf = c[10]; F.csdj = c[11];
+  F.nghl = h->nghl; F.act = act;
+  hipLaunchKernelGGL(k_fields, dim3((h->nghl + 127) / 128, na, 2), dim3(128), 0, h->stream, F);
+  hipLaunchKernelGGL(k_zero_slots, dim3(64, na), dim3(256), 0, h->stream, h->d_hsp.as<double>(), act, 8, 0, 8, h->nxy);
+  ProjArgs P{};
+  P.psi[0] = h->d_wf.as<double>(); P.psi[1] = h->d_wfdr.as<double>(); P.psi[2] = h->d_wfdp.as<double>();
+  P.psi[3] = h->d_wfdz.as<double>(); P.psi[4] = h->d_wfd2.as<double>();
+  P.isstart = h->d_isstart.as<int>(); P.db = h->d_db.as<int>(); P.nsu = h->d_nsu.as<int>(); P.ns = h->d_ns.as<int>();
+  P.flds = h->d_flds.as<double>(); P.hsp = h->d_hsp.as<double>(); P.nxy = h->nxy; P.nghl = h->nghl;
+  P.tiles = h->d_tiles.as<Tile>(); P.act = act;
+  if (h->ntiles) hipLaunchKernelGGL(k_project, dim3(h->ntiles, na), dim3(256), 0, h->stream, P);
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+int stage_sp2qp(pnfam_gpu_ctx *h, int na, const int *act) {
+  const long nxy = h->nxy;
+  hipLaunchKernelGGL(k_zero_slots, dim3(64, na), dim3(256), 0, h->stream, h->d_hqp.as<double>(), act, 4, 0, 4, nxy);
+  for (int ri = 0; ri < 2; ++ri) {
+    if (!h->ngroups_s2q[ri]) continue;
+    hipLaunchKernelGGL(k_triprod, dim3(h->ngroups_s2q[ri], na), dim3(256), h->lds_s2q, h->stream,
+                       h->d_groups_s2q[ri].as<Group>(), h->d_terms_s2q[ri].as<Term>(), h->d_U[0].as<double>(),
+                       h->d_U[1].as<double>(), h->d_U[2].as<double>(), h->d_U[3].as<double>(), h->d_hsp.as<double>(),
+                       8 * nxy, 8, h->d_hqp.as<double>(), 4 * nxy, 4, act, nxy);
+  }
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char *pnfam_gpu_last_error(void) { return g_err.c_str(); }
+
+const char *pnfam_gpu_version(void) { return "pnfam_gpu 0.1 gfx950 fp64"; }
+
+int pnfam_gpu_create(const pnfam_gpu_basis *b, pnfam_gpu_ctx **out) {
+  if (!b || !out) return fail(PNFAM_GPU_EINVAL, "null argument");
+  if (b->nb <= 0 || b->nb % 2 || b->dqp <= 0 || b->nghl <= 0)
+    return fail(PNFAM_GPU_EINVAL, "nb must be positive and even, dqp and nghl positive");
+  std::unique_ptr<pnfam_gpu_ctx> h(new pnfam_gpu_ctx);
+  h->nb = b->nb; h->dqp = b->dqp; h->nghl = b->nghl;
+  h->db.assign(b->db, b->db + b->nb);
+  h->nsu.assign(b->num_spin_up, b->num_spin_up + b->nb);
+  h->ns.assign(b->ns, b->ns + b->dqp);
+  h->isstart.resize(b->nb);
+  int s = 0;
+  for (int i = 0; i < b->nb; ++i) {
+    if (h->db[i] <= 0 || h->nsu[i] < 0 || h->nsu[i] > h->db[i]) return fail(PNFAM_GPU_EINVAL, "bad block size");
+    h->isstart[i] = s;
+    s += h->db[i];
+  }
+  if (s != b->dqp) return fail(PNFAM_GPU_EINVAL, "sum(db) != dqp");
+  for (int i = 0; i < b->nb / 2; ++i)
+    if (h->db[i] != h->db[i + b->nb / 2]) return fail(PNFAM_GPU_EINVAL, "time-reversed blocks must match");
+  for (int i = 0; i < b->dqp; ++i)
+    if (h->ns[i] != 1 && h->ns[i] != -1) return fail(PNFAM_GPU_EINVAL, "ns must be +1 or -1");
+  for (int i = 0; i < b->nb; ++i)   // spin-up states first in each block
+    for (int k = 0; k < h->db[i]; ++k)
+      if ((k < h->nsu[i]) != (h->ns[h->isstart[i] + k] > 0))
+        return fail(PNFAM_GPU_EINVAL, "states must be spin-sorted within blocks");
+  h->cscal.assign(b->cscal, b->cscal + 12);
+  HIP_TRY(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+  const size_t tab = (size_t)b->nghl * b->dqp * 8;
+  size_t dmat = 0;
+  for (int i = 0; i < b->nb; ++i) dmat += (size_t)h->db[i] * h->db[i];
+  int e;
+  if ((e = upload(h->d_wf, b->wf, tab, h->stream)) || (e = upload(h->d_wfdr, b->wfdr, tab, h->stream)) ||
+      (e = upload(h->d_wfdz, b->wfdz, tab, h->stream)) || (e = upload(h->d_wfdp, b->wfdp, tab, h->stream)) ||
+      (e = upload(h->d_wfd2, b->wfd2_all, tab, h->stream)) ||
+      (e = upload(h->d_wdcori, b->wdcori, (size_t)b->nghl * 8, h->stream)) ||
+      (e = upload(h->d_Ep, b->Ep, (size_t)b->dqp * 8, h->stream)) ||
+      (e = upload(h->d_En, b->En, (size_t)b->dqp * 8, h->stream)) ||
+      (e = upload(h->d_U[MAT_UP], b->Up, dmat * 8, h->stream)) ||
+      (e = upload(h->d_U[MAT_VP], b->Vp, dmat * 8, h->stream)) ||
+      (e = upload(h->d_U[MAT_UN], b->Un, dmat * 8, h->stream)) ||
+      (e = upload(h->d_U[MAT_VN], b->Vn, dmat * 8, h->stream)) ||
+      (e = upload(h->d_crho, b->crho, (size_t)b->nghl * 8, h->stream)) ||
+      (e = upload(h->d_cs, b->cs, (size_t)b->nghl * 8, h->stream)) ||
+      (e = upload(h->d_cpair, b->cpair, (size_t)b->nghl * 8, h->stream)) ||
+      (e = upload(h->d_cspair, b->cspair, (size_t)b->nghl * 8, h->stream)) ||
+      (e = upload(h->d_isstart, h->isstart.data(), (size_t)b->nb * 4, h->stream)) ||
+      (e = upload(h->d_db, h->db.data(), (size_t)b->nb * 4, h->stream)) ||
+      (e = upload(h->d_nsu, h->nsu.data(), (size_t)b->nb * 4, h->stream)) ||
+      (e = upload(h->d_ns, h->ns.data(), (size_t)b->dqp * 4, h->stream)))
+    return e;
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  *out = h.release();
+  return 0;
+}
+
+void pnfam_gpu_destroy(pnfam_gpu_ctx *h) { delete h; }
+
+int pnfam_gpu_set_operator(pnfam_gpu_ctx *h, int32_t beta_minus, const int32_t *f_ir2c, const double *f_elem,
+                           int64_t *nxy_out) {
+  if (!h || !f_ir2c || !f_elem) return fail(PNFAM_GPU_EINVAL, "null argument");
+  const int nb = h->nb;
+  std::vector<int> rows(f_ir2c, f_ir2c + nb);
+  std::vector<int> seen(nb, 0);
+  for (int i = 0; i < nb; ++i) {
+    if (rows[i] < -1 || rows[i] >= nb) return fail(PNFAM_GPU_EINVAL, "f_ir2c out of range");
+    if (rows[i] >= 0 && seen[rows[i]]++) return fail(PNFAM_GPU_EINVAL, "f has two blocks in one block column");
+  }
+  h->have_op = false;
+  h->bminus = beta_minus != 0;
+  const std::vector<int> &db = h->db;
+  std::vector<int> diag(nb), tr(nb);
+  for (int i = 0; i < nb; ++i) { diag[i] = i; tr[i] = i < nb / 2 ? i + nb / 2 : i - nb / 2; }
+  BlockMap U = BlockMap::from_rows(db, diag), V = BlockMap::from_rows(db, tr);
+  // V storage follows U's offsets (pnfam_setup.f90:314-320)
+  for (int i = 0; i < nb; ++i) V.ir2m[i] = U.ir2m[i];
+  for (int i = 0; i < nb; ++i) V.ic2m[V.ir2c[i]] = V.ir2m[i];
+  h->fmap = BlockMap::from_rows(db, rows);
+  const long nxy = h->fmap.nelem(db);
+  if (nxy <= 0) return fail(PNFAM_GPU_EINVAL, "operator has no blocks");
+  const int L_U = h->bminus ? MAT_UP : MAT_UN, L_V = h->bminus ? MAT_VP : MAT_VN;
+  const int R_U = h->bminus ? MAT_UN : MAT_UP, R_V = h->bminus ? MAT_VN : MAT_VP;
+  SymBBM WL, WR;
+  WL.m[0] = &U; WL.m[1] = &V; WL.m[2] = &V; WL.m[3] = &U;
+  WL.id[0] = L_U; WL.id[1] = L_V; WL.id[2] = L_V; WL.id[3] = L_U;
+  WR = WL;
+  WR.id[0] = R_U; WR.id[1] = R_V; WR.id[2] = R_V; WR.id[3] = R_U;
+  // Fqp = WL^T Fsp WR : quadrants 12 (s=+1) and 21 (s=-1)
+  SymBBM Fsp;
+  Fsp.m[0] = &h->fmap; Fsp.id[0] = 0;
+  BlockMap fq[4] = {BlockMap(nb), BlockMap(nb), BlockMap(nb), BlockMap(nb)};
+  SymBBM Fqp;
+  Fqp.m[1] = &fq[1]; Fqp.m[2] = &fq[2]; Fqp.id[1] = 0; Fqp.id[2] = 1;
+  Fqp.s[2] = -1; Fqp.s[3] = -1;
+  Planner pf; pf.db = db;
+  BlockMap *fout[4] = {&fq[0], &fq[1], &fq[2], &fq[3]};
+  if (!pf.product('t', WL, 'n', Fsp, 'n', WR, Fqp, fout)) return fail(PNFAM_GPU_EINVAL, pf.err);
+  h->xmap = fq[1];
+  h->ymap = fq[2];
+  if (h->xmap.nelem(db) != nxy || h->ymap.nelem(db) != nxy)
+    return fail(PNFAM_GPU_EINVAL, "operator block structure not supported (X/Y sizes differ from f)");
+  // dR_sp = WL dR_qp WR^T, per re/im: quadrant signs/transposes of pnfam_solver.f90:354-363
+  for (int ri = 0; ri < 2; ++ri) {
+    SymBBM dR;
+    dR.m[1] = &h->xmap; dR.m[2] = &h->ymap;
+    dR.id[1] = 2 * ri; dR.id[2] = 2 * ri + 1;   // vin slots: ReX ReY ImX ImY
+    dR.s[2] = -1; dR.s[3] = -1;
+    dR.imag = ri == 1;
+    SymBBM out;
+    BlockMap om[4] = {BlockMap(nb), BlockMap(nb), BlockMap(nb), BlockMap(nb)};
+    for (int q = 0; q < 4; ++q) { out.m[q] = &om[q]; out.id[q] = 2 * q + ri; }
+    if (ri == 0) { out.s[1] = -1; out.s[3] = -1; } else { out.s[1] = -1; out.s[2] = -1; out.s[3] = -1; }
+    out.t[1] = out.t[2] = out.t[3] = 't';
+    out.imag = ri == 1;
+    Planner p; p.db = db;
+    BlockMap *omp[4] = {&om[0], &om[1], &om[2], &om[3]};
+    if (!p.product('n', WL, 'n', dR, 't', WR, out, omp)) return fail(PNFAM_GPU_EINVAL, p.err);
+    if (ri == 0) for (int q = 0; q < 4; ++q) h->rspmap[q] = om[q];
+    for (int q = 0; q < 4; ++q)
+      if (om[q].nelem(db) > nxy) return fail(PNFAM_GPU_EINVAL, "density quadrant larger than nxy");
+    std::vector<Term> terms; std::vector<Group> groups;
+    p.finish(terms, groups);
+    size_t lds = 0;
+    for (auto &g : groups) lds = std::max(lds, (size_t)g.nrow * g.ncol * 8);
+    h->lds_q2s = std::max(h->lds_q2s, lds);
+    int e;
+    if ((e = upload(h->d_terms_q2s[ri], terms.data(), std::max<size_t>(1, terms.size()) * sizeof(Term), h->stream)) ||
+        (e = upload(h->d_groups_q2s[ri], groups.data(), std::max<size_t>(1, groups.size()) * sizeof(Group), h->stream)))
+      return e;
+    h->ngroups_q2s[ri] = (int)groups.size();
+  }
+  // dH_sp structures preset as the reference (pnfam_solver.f90:402-413):
+  // h_pn, h_np like f; Delta+ like X; Delta- like Y
+  h->hspmap[0] = h->fmap; h->hspmap[1] = h->xmap; h->hspmap[2] = h->ymap; h->hspmap[3] = h->fmap;
+  for (int ri = 0; ri < 2; ++ri) {
+    SymBBM dH;
+    for (int q = 0; q < 4; ++q) { dH.m[q] = &h->hspmap[q]; dH.id[q] = 2 * q + ri; }
+    if (ri == 0) { dH.s[2] = -1; dH.s[3] = -1; } else { dH.s[3] = -1; }
+    dH.t[3] = 't';
+    dH.imag = ri == 1;
+    SymBBM out;
+    BlockMap om[4] = {BlockMap(nb), BlockMap(nb), BlockMap(nb), BlockMap(nb)};
+    out.m[1] = &om[1]; out.m[2] = &om[2];
+    out.id[1] = ri == 0 ? 0 : 2; out.id[2] = ri == 0 ? 1 : 3;   // hqp slots
+    out.s[2] = -1; out.s[3] = -1;
+    out.imag = ri == 1;
+    Planner p; p.db = db;
+    BlockMap *omp[4] = {&om[0], &om[1], &om[2], &om[3]};
+    if (!p.product('t', WL, 'n', dH, 'n', WR, out, omp)) return fail(PNFAM_GPU_EINVAL, p.err);
+    for (int i = 0; i < nb; ++i)
+      if (om[1].ir2c[i] != h->xmap.ir2c[i] || om[2].ir2c[i] != h->ymap.ir2c[i])
+        return fail(PNFAM_GPU_EINVAL, "dH_qp structure differs from dR_qp");
+    std::vector<Term> terms; std::vector<Group> groups;
+    p.finish(terms, groups);
+    size_t lds = 0;
+    for (auto &g : groups) lds = std::max(lds, (size_t)g.nrow * g.ncol * 8);
+    h->lds_s2q = std::max(h->lds_s2q, lds);
+    int e;
+    if ((e = upload(h->d_terms_s2q[ri], terms.data(), std::max<size_t>(1, terms.size()) * sizeof(Term), h->stream)) ||
+        (e = upload(h->d_groups_s2q[ri], groups.data(), std::max<size_t>(1, groups.size()) * sizeof(Group), h->stream)))
+      return e;
+    h->ngroups_s2q[ri] = (int)groups.size();
+  }
+  // density pair lists: sector 0 (rho_pn = m11, kappa+ = m12), sector 1 (rho_np = m22, kappa- = m21)
+  std::vector<int4> pl;
+  const BlockMap *pm[4] = {&h->rspmap[0], &h->rspmap[1], &h->rspmap[3], &h->rspmap[2]};
+  for (int k = 0; k < 4; ++k) {
+    auto v = pair_list(*pm[k]);
+    h->pair_off[k] = (int)pl.size();
+    h->pair_n[k] = (int)v.size();
+    pl.insert(pl.end(), v.begin(), v.end());
+  }
+  int e;
+  if ((e = upload(h->d_pairs, pl.data(), std::max<size_t>(1, pl.size()) * sizeof(int4), h->stream))) return e;
+  // projection tiles: h_pn (slot 0, sector 0), Delta+ (slot 1, sector 0), Delta- (slot 2, sector 1), h_np (slot 3, sector 1)
+  std::vector<Tile> tiles;
+  const int slot_sec[4] = {0, 0, 1, 1};
+  for (int q = 0; q < 4; ++q) {
+    for (auto &p : pair_list(h->hspmap[q])) {
+      const int dx = db[p.x], dy = db[p.y];
+      for (int a0 = 0; a0 < dx; a0 += PT)
+        for (int b0 = 0; b0 < dy; b0 += PT) {
+          Tile t{};
+          t.ix = p.x; t.iy = p.y; t.off = p.z; t.sec = slot_sec[q];
+          t.a0 = a0; t.b0 = b0; t.na = std::min(PT, dx - a0); t.nb_ = std::min(PT, dy - b0);
+          t.out_slot = q; t.kind = (q == 1 || q == 2) ? 1 : 0;
+          tiles.push_back(t);
+        }
+    }
+  }
+  h->ntiles = (int)tiles.size();
+  if ((e = upload(h->d_tiles, tiles.data(), std::max<size_t>(1, tiles.size()) * sizeof(Tile), h->stream))) return e;
+  // element -> (row state, column state) for the Green's function
+  auto rc = [&](const BlockMap &m, std::vector<int> &row, std::vector<int> &col) {
+    row.assign(nxy, 0); col.assign(nxy, 0);
+    for (int i = 0; i < nb; ++i) {
+      int j = m.ir2c[i];
+      if (j < 0) continue;
+      for (int c = 0; c < db[j]; ++c)
+        for (int r = 0; r < db[i]; ++r) {
+          long e2 = m.ir2m[i] + r + (long)c * db[i];
+          row[e2] = h->isstart[i] + r;
+          col[e2] = h->isstart[j] + c;
+        }
+    }
+  };
+  std::vector<int> xr, xc, yr, yc;
+  rc(h->xmap, xr, xc);
+  rc(h->ymap, yr, yc);
+  if ((e = upload(h->d_xrow, xr.data(), nxy * 4, h->stream)) || (e = upload(h->d_xcol, xc.data(), nxy * 4, h->stream)) ||
+      (e = upload(h->d_yrow, yr.data(), nxy * 4, h->stream)) || (e = upload(h->d_ycol, yc.data(), nxy * 4, h->stream)))
+    return e;
+  // Fqp on the device with the same product machinery (one "frequency": B = f)
+  h->nxy = nxy;
+  {
+    Planner p; p.db = db;
+    SymBBM F1 = Fsp;
+    BlockMap om[4] = {BlockMap(nb), BlockMap(nb), BlockMap(nb), BlockMap(nb)};
+    SymBBM out = Fqp;
+    out.m[1] = &om[1]; out.m[2] = &om[2];
+    BlockMap *omp[4] = {&om[0], &om[1], &om[2], &om[3]};
+    if (!p.product('t', WL, 'n', F1, 'n', WR, out, omp)) return fail(PNFAM_GPU_EINVAL, p.err);
+    std::vector<Term> terms; std::vector<Group> groups;
+    p.finish(terms, groups);
+    DevPtr dt, dg, df, dout, dact;
+    size_t lds = 8;
+    for (auto &g : groups) lds = std::max(lds, (size_t)g.nrow * g.ncol * 8);
+    int zero = 0;
+    if ((e = upload(dt, terms.data(), std::max<size_t>(1, terms.size()) * sizeof(Term), h->stream)) ||
+        (e = upload(dg, groups.data(), std::max<size_t>(1, groups.size()) * sizeof(Group), h->stream)) ||
+        (e = upload(df, f_elem, nxy * 8, h->stream)) || (e = dout.alloc(2 * nxy * 8)) ||
+        (e = upload(dact, &zero, 4, h->stream)))
+      return e;
+    hipLaunchKernelGGL(k_fill, grid1(2 * nxy, 256), dim3(256), 0, h->stream, dout.as<double>(), (size_t)(2 * nxy), 0.0);
+    if (!groups.empty())
+      hipLaunchKernelGGL(k_triprod, dim3((unsigned)groups.size(), 1), dim3(256), lds, h->stream, dg.as<Group>(),
+                         dt.as<Term>(), h->d_U[0].as<double>(), h->d_U[1].as<double>(), h->d_U[2].as<double>(),
+                         h->d_U[3].as<double>(), df.as<double>(), 0L, 1, dout.as<double>(), 0L, 2, dact.as<int>(), nxy);
+    HIP_TRY(hipGetLastError());
+    if ((e = h->d_f20.alloc(nxy * 8)) || (e = h->d_f02.alloc(nxy * 8))) return e;
+    HIP_TRY(hipMemcpyAsync(h->d_f20.p, dout.as<double>(), nxy * 8, hipMemcpyDeviceToDevice, h->stream));
+    HIP_TRY(hipMemcpyAsync(h->d_f02.p, dout.as<double>() + nxy, nxy * 8, hipMemcpyDeviceToDevice, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+  }
+  h->have_op = true;
+  h->cap = 0;
+  h->hist_cap = -1;
+  if (nxy_out) *nxy_out = nxy;
+  return 0;
+}
+
+int pnfam_gpu_structures(pnfam_gpu_ctx *h, int32_t *maps, int32_t *xy_maps) {
+  if (!h || !h->have_op) return fail(PNFAM_GPU_ESTATE, "no operator set");
+  for (int q = 0; q < 4; ++q)
+    if (maps) std::copy(h->rspmap[q].ir2c.begin(), h->rspmap[q].ir2c.end(), maps + q * h->nb);
+  if (xy_maps) {
+    std::copy(h->xmap.ir2c.begin(), h->xmap.ir2c.end(), xy_maps);
+    std::copy(h->ymap.ir2c.begin(), h->ymap.ir2c.end(), xy_maps + h->nb);
+  }
+  return 0;
+}
+
+int pnfam_gpu_get_fqp(pnfam_gpu_ctx *h, double *f20, double *f02) {
+  if (!h || !h->have_op) return fail(PNFAM_GPU_ESTATE, "no operator set");
+  HIP_TRY(hipMemcpy(f20, h->d_f20.p, h->nxy * 8, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(f02, h->d_f02.p, h->nxy * 8, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+static int one_point(pnfam_gpu_ctx *h) {
+  if (!h || !h->have_op) return fail(PNFAM_GPU_ESTATE, "no operator set");
+  if (int e = ensure_workspace(h, 1, h->hist_cap < 0 ? 1 : h->hist_cap)) return e;
+  int zero = 0;
+  HIP_TRY(hipMemcpyAsync(h->d_act.p, &zero, 4, hipMemcpyHostToDevice, h->stream));
+  return 0;
+}
+
+int pnfam_gpu_qp2sp(pnfam_gpu_ctx *h, const double *x_re, const double *x_im, const double *y_re, const double *y_im,
+                    double *rsp) {
+  if (int e = one_point(h)) return e;
+  const long n = h->nxy;
+  double *v = h->d_vin.as<double>();
+  HIP_TRY(hipMemcpyAsync(v, x_re, n * 8, hipMemcpyHostToDevice, h->stream));
+  HIP_TRY(hipMemcpyAsync(v + n, y_re, n * 8, hipMemcpyHostToDevice, h->stream));
+  HIP_TRY(hipMemcpyAsync(v + 2 * n, x_im, n * 8, hipMemcpyHostToDevice, h->stream));
+  HIP_TRY(hipMemcpyAsync(v + 3 * n, y_im, n * 8, hipMemcpyHostToDevice, h->stream));
+  if (int e = stage_qp2sp(h, v, 1, h->d_act.as<int>())) return e;
+  HIP_TRY(hipMemcpyAsync(rsp, h->d_rsp.p, 8 * n * 8, hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+// The 16-array argument order of calc_dHsp_interface (rho_pn, kappa+, rho_np,
+// kappa-; h_pn, Delta+, h_np, Delta-) maps to quadrant slots m11, m12, m22, m21.
+static const int kArgSlot[8] = {0, 1, 2, 3, 6, 7, 4, 5};
+
+static int upload_args(pnfam_gpu_ctx *h, DevPtr &dst, const double *in) {
+  const long n = h->nxy;
+  for (int k = 0; k < 8; ++k)
+    HIP_TRY(hipMemcpyAsync(dst.as<double>() + (size_t)kArgSlot[k] * n, in + (size_t)k * n, n * 8,
+                           hipMemcpyHostToDevice, h->stream));
+  return 0;
+}
+
+int pnfam_gpu_calc_hamiltonian(pnfam_gpu_ctx *h, const double *in, double *out) {
+  if (int e = one_point(h)) return e;
+  const long n = h->nxy;
+  if (int e = upload_args(h, h->d_rsp, in)) return e;
+  if (int e = stage_hamiltonian(h, 1, h->d_act.as<int>())) return e;
+  for (int k = 0; k < 8; ++k)
+    HIP_TRY(hipMemcpyAsync(out + (size_t)k * n, h->d_hsp.as<double>() + (size_t)kArgSlot[k] * n, n * 8,
+                           hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int pnfam_gpu_densities(pnfam_gpu_ctx *h, const double *in, int32_t sector, double *dens) {
+  if (sector != 0 && sector != 1) return fail(PNFAM_GPU_EINVAL, "sector must be 0 or 1");
+  if (int e = one_point(h)) return e;
+  if (int e = upload_args(h, h->d_rsp, in)) return e;
+  if (int e = stage_hamiltonian(h, 1, h->d_act.as<int>())) return e;
+  HIP_TRY(hipMemcpyAsync(dens, h->d_dens.as<double>() + (size_t)sector * 2 * NDENS * h->nghl,
+                         (size_t)2 * NDENS * h->nghl * 8, hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int pnfam_gpu_sp2qp(pnfam_gpu_ctx *h, const double *hsp, double *h20_re, double *h20_im, double *h02_re,
+                    double *h02_im) {
+  if (int e = one_point(h)) return e;
+  const long n = h->nxy;
+  if (int e = upload_args(h, h->d_hsp, hsp)) return e;
+  if (int e = stage_sp2qp(h, 1, h->d_act.as<int>())) return e;
+  const double *q = h->d_hqp.as<double>();
+  HIP_TRY(hipMemcpyAsync(h20_re, q, n * 8, hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(hipMemcpyAsync(h02_re, q + n, n * 8, hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(hipMemcpyAsync(h20_im, q + 2 * n, n * 8, hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(hipMemcpyAsync(h02_im, q + 3 * n, n * 8, hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int pnfam_gpu_solve_batch(pnfam_gpu_ctx *h, int32_t npts, const double *re_w, const double *im_w,
+                          const pnfam_gpu_opts *o, double *re_s, double *im_s, int32_t *iters, double *si_final,
+                          double *x_out) {
+  if (!h || !o || !re_w || !im_w || !re_s || !im_s || !iters) return fail(PNFAM_GPU_EINVAL, "null argument");
+  if (!h->have_op) return fail(PNFAM_GPU_ESTATE, "no operator set");
+  if (npts <= 0) return fail(PNFAM_GPU_EINVAL, "npts must be positive");
+  if (o->max_iter < 0) return fail(PNFAM_GPU_EINVAL, "max_iter must be >= 0");
+  if (o->precision != PNFAM_GPU_FP64) return fail(PNFAM_GPU_EINVAL, "only the FP64 path is built");
+  int M = o->broyden_history;
+  const bool skip_res = std::fabs(o->quench) < 1e-10;
+  if (skip_res) M = -1;   // no residual interaction: no mixing (pnfam_solver.f90:261-264)
+  if (M > BRO_MAXM) return fail(PNFAM_GPU_EINVAL, "broyden_history above " + std::to_string(BRO_MAXM));
+  const long nxy = h->nxy, N = 4 * nxy;
+  if (int e = ensure_workspace(h, npts, std::max(M, 0))) return e;
+  hipStream_t st = h->stream;
+  HIP_TRY(hipMemcpyAsync(h->d_wre.p, re_w, npts * 8, hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(h->d_wim.p, im_w, npts * 8, hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_fill, grid1((size_t)npts * N, 256), dim3(256), 0, st, h->d_vin.as<double>(), (size_t)npts * N, 0.0);
+  if (M > 0)
+    hipLaunchKernelGGL(k_fill, grid1((size_t)npts * M * M, 256), dim3(256), 0, st, h->d_gram.as<double>(),
+                       (size_t)npts * M * M, 0.0);
+  const double *e1 = h->bminus ? h->d_Ep.as<double>() : h->d_En.as<double>();
+  const double *e2 = h->bminus ? h->d_En.as<double>() : h->d_Ep.as<double>();
+  std::vector<double> si(npts, 1.0), S(2 * npts, 0.0);
+  std::vector<int> done(npts, 0), act;
+  for (int w = 0; w < npts; ++w) { iters[w] = -1; re_s[w] = 0.0; im_s[w] = 0.0; }
+  const int nwg = h->bro_nwg;
+  for (int iter = 0; iter <= o->max_iter; ++iter) {
+    act.clear();
+    for (int w = 0; w < npts; ++w) {
+      if (done[w]) continue;
+      if (si[w] < o->convergence_epsilon) { done[w] = 1; iters[w] = iter; continue; }
+      if (iter == o->max_iter) { done[w] = 1; iters[w] = -1; continue; }
+      act.push_back(w);
+    }
+    if (act.empty()) break;
+    const int na = (int)act.size();
+    int *dact = h->d_act.as<int>();
+    HIP_TRY(hipMemcpyAsync(dact, act.data(), na * 4, hipMemcpyHostToDevice, st));
+    if (!skip_res) {
+      if (int e = stage_qp2sp(h, h->d_vin.as<double>(), na, dact)) return e;
+      if (int e = stage_hamiltonian(h, na, dact)) return e;
+      if (int e = stage_sp2qp(h, na, dact)) return e;
+    }
+    GreenArgs G{};
+    G.hqp = h->d_hqp.as<double>(); G.f20 = h->d_f20.as<double>(); G.f02 = h->d_f02.as<double>();
+    G.e1 = e1; G.e2 = e2;
+    G.xrow = h->d_xrow.as<int>(); G.xcol = h->d_xcol.as<int>(); G.yrow = h->d_yrow.as<int>(); G.ycol = h->d_ycol.as<int>();
+    G.wre = h->d_wre.as<double>(); G.wim = h->d_wim.as<double>(); G.vout = h->d_vout.as<double>();
+    G.nxy = nxy; G.quench = o->quench; G.use_h = !skip_res; G.act = dact;
+    hipLaunchKernelGGL(k_green, dim3((unsigned)std::min<long>(1024, (2 * nxy + 255) / 256), na), dim3(256), 0, st, G);
+    // Broyden
+    BroArgs B{};
+    B.vin = h->d_vin.as<double>(); B.vout = h->d_vout.as<double>(); B.df = h->d_df.as<double>();
+    B.dv = h->d_dv.as<double>(); B.part = h->d_part.as<double>(); B.gram = h->d_gram.as<double>();
+    B.coef = h->d_coef.as<double>(); B.si = h->d_si.as<double>(); B.scal = h->d_scal.as<double>();
+    B.N = N; B.M = std::max(M, 0); B.nwg = nwg; B.iter = iter; B.alpha = o->alpha_mix; B.act = dact;
+    int mode = 0;
+    if (M < 0) mode = 2;
+    else if (M == 0 || iter == 0) mode = 1;
+    if (mode == 0) {
+      B.iter_used = std::min(iter - 1, M);
+      B.ipos = (iter - 1 - ((iter - 2) / M) * M) - 1;   // 0-based slots
+      B.inext = (iter - ((iter - 1) / M) * M) - 1;
+      if (iter < 2) B.ipos = 0;
+    }
+    B.iter = mode == 0 ? iter : 0;
+    hipLaunchKernelGGL(k_bro_resid, dim3(nwg, na), dim3(BRO_WG), 0, st, B);
+    hipLaunchKernelGGL(k_bro_norm, dim3(na), dim3(64), 0, st, B);
+    if (mode == 0) {
+      hipLaunchKernelGGL(k_bro_dots, dim3(nwg, na), dim3(BRO_WG), 0, st, B);
+      hipLaunchKernelGGL(k_bro_solve, dim3(na), dim3(256), 0, st, B);
+    }
+    hipLaunchKernelGGL(k_bro_update, dim3(nwg, na), dim3(BRO_WG), 0, st, B, mode);
+    const int cnwg = 64;
+    hipLaunchKernelGGL(k_contract, dim3(cnwg, na), dim3(256), 0, st, h->d_vin.as<double>(), h->d_f20.as<double>(),
+                       h->d_f02.as<double>(), nxy, h->d_cpart.as<double>(), cnwg, dact);
+    hipLaunchKernelGGL(k_contract_fin, dim3(na), dim3(64), 0, st, h->d_cpart.as<double>(), cnwg, h->d_S.as<double>(), dact);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(si.data(), h->d_si.p, npts * 8, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(S.data(), h->d_S.p, 2 * npts * 8, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    for (int w : act) { re_s[w] = S[2 * w]; im_s[w] = S[2 * w + 1]; }
+  }
+  if (si_final) for (int w = 0; w < npts; ++w) si_final[w] = si[w];
+  if (x_out) HIP_TRY(hipMemcpy(x_out, h->d_vin.p, (size_t)npts * N * 8, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,193 @@
+"""Host side of the MI355X pnFAM engine: ctypes binding of libpnfam_gpu.so
+(include/pnfam_gpu.h) and a Python object mirroring what pnfam_solve does for
+one (operator, K) -- set-up once, then solve any number of frequencies.
+
+There is no CPU fallback: if the HIP library is missing or the device call
+fails, the error is raised.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from .problem import Problem
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpnfam_gpu.so")
+
+# The reference's Broyden mixing factor is a default-real (single-precision)
+# literal 0.7 promoted to double (pnfam_broyden.f90:20).
+ALPHA_MIX = float(np.float32(0.7))
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+
+
+class _Basis(C.Structure):
+    _fields_ = [("nb", C.c_int32), ("dqp", C.c_int32), ("nghl", C.c_int32),
+                ("db", _ip), ("num_spin_up", _ip), ("ns", _ip),
+                ("wf", _dp), ("wfdr", _dp), ("wfdz", _dp), ("wfdp", _dp), ("wfd2_all", _dp),
+                ("wdcori", _dp), ("Ep", _dp), ("En", _dp),
+                ("Up", _dp), ("Vp", _dp), ("Un", _dp), ("Vn", _dp),
+                ("crho", _dp), ("cs", _dp), ("cpair", _dp), ("cspair", _dp), ("cscal", _dp)]
+
+
+class _Opts(C.Structure):
+    _fields_ = [("max_iter", C.c_int32), ("convergence_epsilon", C.c_double), ("broyden_history", C.c_int32),
+                ("quench", C.c_double), ("alpha_mix", C.c_double), ("precision", C.c_int32)]
+
+
+FP64 = 0
+MIXED = 1
+
+
+class GpuError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load_library(path=LIB_PATH):
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise GpuError(f"{path} not built: run `python -m pynfam_amd.csrc.build` (or __graft_entry__.build())")
+    lib = C.CDLL(path)
+    P = C.c_void_p
+    lib.pnfam_gpu_create.argtypes = [C.POINTER(_Basis), C.POINTER(P)]
+    lib.pnfam_gpu_set_operator.argtypes = [P, C.c_int32, _ip, _dp, C.POINTER(C.c_int64)]
+    lib.pnfam_gpu_solve_batch.argtypes = [P, C.c_int32, _dp, _dp, C.POINTER(_Opts), _dp, _dp, _ip, _dp, _dp]
+    lib.pnfam_gpu_structures.argtypes = [P, _ip, _ip]
+    lib.pnfam_gpu_get_fqp.argtypes = [P, _dp, _dp]
+    lib.pnfam_gpu_qp2sp.argtypes = [P, _dp, _dp, _dp, _dp, _dp]
+    lib.pnfam_gpu_calc_hamiltonian.argtypes = [P, _dp, _dp]
+    lib.pnfam_gpu_sp2qp.argtypes = [P, _dp, _dp, _dp, _dp, _dp]
+    lib.pnfam_gpu_densities.argtypes = [P, _dp, C.c_int32, _dp]
+    lib.pnfam_gpu_destroy.argtypes = [P]
+    lib.pnfam_gpu_destroy.restype = None
+    lib.pnfam_gpu_last_error.restype = C.c_char_p
+    lib.pnfam_gpu_version.restype = C.c_char_p
+    for name in ("pnfam_gpu_create", "pnfam_gpu_set_operator", "pnfam_gpu_solve_batch", "pnfam_gpu_structures",
+                 "pnfam_gpu_get_fqp", "pnfam_gpu_qp2sp", "pnfam_gpu_calc_hamiltonian", "pnfam_gpu_sp2qp",
+                 "pnfam_gpu_densities"):
+        getattr(lib, name).restype = C.c_int
+    _lib = lib
+    return lib
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    return a.ctypes.data_as(_ip)
+
+
+class GpuEngine:
+    """One device context with one operator set.  Mirrors pnfam_solve's set-up
+    (pnfam_solver.f90:32-59) once and its ifam loop for batches of points."""
+
+    def __init__(self, prob: Problem):
+        self.lib = load_library()
+        self.prob = prob
+        b, c = prob.basis, prob.couplings
+        self._keep = []
+
+        def dd(a):
+            a = np.ascontiguousarray(a, dtype=np.float64) if a.ndim == 1 else np.asfortranarray(a, dtype=np.float64)
+            self._keep.append(a)
+            return _d(a)
+
+        def ii(a):
+            a = np.ascontiguousarray(a, dtype=np.int32)
+            self._keep.append(a)
+            return _i(a)
+
+        basis = _Basis(nb=b.nb, dqp=b.dqp, nghl=b.nghl, db=ii(b.db), num_spin_up=ii(b.num_spin_up), ns=ii(b.ns),
+                       wf=dd(b.wf), wfdr=dd(b.wfdr), wfdz=dd(b.wfdz), wfdp=dd(b.wfdp), wfd2_all=dd(b.wfd2_all),
+                       wdcori=dd(b.grid.wdcori), Ep=dd(b.Ep), En=dd(b.En), Up=dd(b.Up), Vp=dd(b.Vp),
+                       Un=dd(b.Un), Vn=dd(b.Vn), crho=dd(c.crho), cs=dd(c.cs), cpair=dd(c.cpair),
+                       cspair=dd(c.cspair), cscal=dd(c.scalars()))
+        self.h = C.c_void_p()
+        self._check(self.lib.pnfam_gpu_create(C.byref(basis), C.byref(self.h)))
+        self._keep = []
+        nxy = C.c_int64()
+        fmap = np.ascontiguousarray(prob.f_struct.ir2c, dtype=np.int32)
+        felem = np.ascontiguousarray(prob.f_elem, dtype=np.float64)
+        self._check(self.lib.pnfam_gpu_set_operator(self.h, int(prob.bminus), _i(fmap), _d(felem), C.byref(nxy)))
+        self.nxy = int(nxy.value)
+
+    def _check(self, rc):
+        if rc != 0:
+            raise GpuError(f"pnfam_gpu error {rc}: {self.lib.pnfam_gpu_last_error().decode()}")
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            self.lib.pnfam_gpu_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- batched solve -----------------------------------------------------
+    def solve(self, omegas, max_iter=300, eps=1e-7, broyden_history=50, quench=None, alpha_mix=ALPHA_MIX,
+              precision=FP64, want_x=False):
+        w = np.atleast_1d(np.asarray(omegas, dtype=np.complex128))
+        n = w.shape[0]
+        re, im = np.ascontiguousarray(w.real), np.ascontiguousarray(w.imag)
+        q = self.prob.quench if quench is None else quench
+        o = _Opts(max_iter=max_iter, convergence_epsilon=eps, broyden_history=broyden_history, quench=q,
+                  alpha_mix=alpha_mix, precision=precision)
+        rs, is_, si = np.zeros(n), np.zeros(n), np.zeros(n)
+        it = np.zeros(n, dtype=np.int32)
+        x = np.zeros((n, 4 * self.nxy)) if want_x else None
+        self._check(self.lib.pnfam_gpu_solve_batch(self.h, n, _d(re), _d(im), C.byref(o), _d(rs), _d(is_), _i(it),
+                                                   _d(si), _d(x) if want_x else None))
+        out = {"strength": rs + 1j * is_, "iterations": it, "si": si}
+        if want_x:
+            out["x"] = x
+        return out
+
+    # ---- per-unit entry points ---------------------------------------------
+    def structures(self):
+        nb = self.prob.basis.nb
+        maps = np.zeros((4, nb), dtype=np.int32)
+        xy = np.zeros((2, nb), dtype=np.int32)
+        self._check(self.lib.pnfam_gpu_structures(self.h, _i(maps), _i(xy)))
+        return maps, xy
+
+    def fqp(self):
+        f20, f02 = np.zeros(self.nxy), np.zeros(self.nxy)
+        self._check(self.lib.pnfam_gpu_get_fqp(self.h, _d(f20), _d(f02)))
+        return f20, f02
+
+    def qp2sp(self, x_re, x_im, y_re, y_im):
+        rsp = np.zeros((8, self.nxy))
+        a = [np.ascontiguousarray(v, dtype=np.float64) for v in (x_re, x_im, y_re, y_im)]
+        self._check(self.lib.pnfam_gpu_qp2sp(self.h, *[_d(v) for v in a], _d(rsp)))
+        return rsp
+
+    def calc_hamiltonian(self, rsp):
+        rsp = np.ascontiguousarray(rsp, dtype=np.float64)
+        out = np.zeros((8, self.nxy))
+        self._check(self.lib.pnfam_gpu_calc_hamiltonian(self.h, _d(rsp), _d(out)))
+        return out
+
+    def densities(self, rsp, sector):
+        rsp = np.ascontiguousarray(rsp, dtype=np.float64)
+        dens = np.zeros((56, self.prob.basis.nghl))
+        self._check(self.lib.pnfam_gpu_densities(self.h, _d(rsp), int(sector), _d(dens)))
+        return dens
+
+    def sp2qp(self, hsp):
+        hsp = np.ascontiguousarray(hsp, dtype=np.float64)
+        out = [np.zeros(self.nxy) for _ in range(4)]
+        self._check(self.lib.pnfam_gpu_sp2qp(self.h, _d(hsp), *[_d(v) for v in out]))
+        return out   # h20_re, h20_im, h02_re, h02_im

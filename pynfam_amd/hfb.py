@@ -536,6 +536,11 @@ def load_pnfam_basis(hel_path) -> PnfamBasis:
     grid, qhla, fi1r, fi1z, fi2d = ho_wavefunctions(h)
     mats = hfb_matrices(h, grid, qhla, fi1r, fi1z, fi2d)
     sols = hfb_diagonalise(h, grid, mats, qhla)
+    return assemble_pnfam_basis(h, grid, qhla, fi1r, fi1z, fi2d, sols)
+
+
+def assemble_pnfam_basis(h, grid, qhla, fi1r, fi1z, fi2d, sols) -> PnfamBasis:
+    """Doubling, spin sort and pairing-window cut (hfbtho_solution.f90:162-459)."""
     nbh = h.nb
     nb = 2 * nbh
     dh = h.ID.astype(np.int64)
