@@ -1,0 +1,146 @@
+"""Throughput of the batched pnFAM iteration on one MI355X.
+
+A step solves one batch of contour frequencies to convergence for each of
+GT- K=0 and K=1 on the N_sh=16, Z=60 N=92, beta2=0.3, 40x40-grid workload
+(synthetic, structurally exact HFB input; see pynfam_amd/synthetic.py).
+value = converged frequency points per second over the timed steps (a point
+counts when its max-abs residual drops below 1e-7; points reaching max_iter
+count as failures).  Inputs are resident in HBM before timing starts.
+
+The JSON line also carries the dominant kernel's achieved rate against the
+MI355X FP64 peak (HIP events around every launch on the engine stream during
+the timed steps) and the CPU oracle timed on a bounded sample of the same
+workload on this host.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "converged FAM freq points/sec at N_sh=16 (mixed vs FP64 GPU), rate err vs FP64"
+PEAK_FP64_TFLOPS = 78.6
+PEAK_HBM_GBS = 8000.0
+
+
+def contour(npts, center=8.0, radius=10.0):
+    """Upper half of a circle through [center-radius, center+radius] with
+    Gauss-Legendre nodes in the angle (the closed-contour half that is computed,
+    the other half follows from S(w*) = S(w)*)."""
+    import numpy as np
+    x, _ = np.polynomial.legendre.leggauss(2 * npts)
+    th = np.pi * (x[x > 0] + 0.0)                      # (0, pi)
+    return center + radius * np.exp(1j * th)
+
+
+def build_workload(nsh, beta, Z, N, ngrid):
+    from pynfam_amd.problem import build_problem
+    from pynfam_amd.synthetic import synthetic_basis
+    basis = synthetic_basis(Z=Z, N=N, n00=nsh, beta=beta, ngh=ngrid, ngl=ngrid)
+    probs = [build_problem(basis, "GT", K, "-", "SKOP", vpair_t0=-346.352, override_cs0=128.279,
+                           override_csr=0.0, override_cds=0.0) for K in (0, 1)]
+    return basis, probs
+
+
+def cpu_baseline(prob, mean_iters, iters_sample, threads):
+    """C oracle on one frequency for a few iterations; seconds per iteration
+    scaled to points/s with the GPU run's mean iteration count."""
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    from oracle.oracle import Oracle
+    o = Oracle(prob)
+    w = complex(8.0, 3.0)
+    t0 = time.perf_counter()
+    o.ifam(w, max_iter=iters_sample)
+    dt = time.perf_counter() - t0
+    s_per_iter = dt / iters_sample
+    return {"value": 1.0 / (s_per_iter * mean_iters), "unit": "points/s", "cores": threads, "kind": "port",
+            "sample": f"oracle (C, FP64, OpenMP) on one K=0 point, {iters_sample} iterations in {dt:.1f} s "
+                      f"= {s_per_iter:.2f} s/iteration, scaled by the GPU mean of {mean_iters:.1f} iterations/point"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--nsh", type=int, default=16)
+    ap.add_argument("--beta", type=float, default=0.3)
+    ap.add_argument("--ngrid", type=int, default=40)
+    ap.add_argument("--npts", type=int, default=128, help="frequencies per K per step")
+    ap.add_argument("--max-iter", type=int, default=300)
+    ap.add_argument("--cpu-iters", type=int, default=2, help="oracle iterations in the CPU sample (0: skip)")
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this engine runs one frequency batch per GPU; only --gpus 1 is measured "
+                                   "(points are independent: N GPUs would each take npts/N points)"}))
+        sys.exit(1)
+
+    import numpy as np
+    from pynfam_amd.engine import GpuEngine
+
+    basis, probs = build_workload(args.nsh, args.beta, 60, 92, args.ngrid)
+    engines = [GpuEngine(p) for p in probs]
+    w = contour(args.npts)
+
+    def step():
+        conv, its = 0, []
+        for e in engines:
+            r = e.solve(w, max_iter=args.max_iter)
+            ok = r["iterations"] >= 0
+            conv += int(ok.sum())
+            its.extend(r["iterations"][ok].tolist())
+        return conv, its
+
+    for _ in range(args.warmup):
+        step()
+    for e in engines:
+        e.profile(True)
+    t0 = time.perf_counter()
+    conv_total, its_all = 0, []
+    for _ in range(args.steps):
+        c, its = step()
+        conv_total += c
+        its_all.extend(its)
+    dt = time.perf_counter() - t0
+    prof = {}
+    for e in engines:
+        for k, v in e.profile_read().items():
+            d = prof.setdefault(k, {"ms": 0.0, "launches": 0, "units": 0.0, "flops": 0.0})
+            d["ms"] += v["ms"]
+            d["launches"] += v["launches"]
+            d["units"] += v["units"]
+            d["flops"] += v["flops_per_unit"] * v["units"]
+        e.profile(False)
+    total_pts = args.steps * len(engines) * len(w)
+    value = conv_total / dt
+    mean_iters = float(np.mean(its_all)) if its_all else float("nan")
+    dom = max(prof.items(), key=lambda kv: kv[1]["ms"])
+    name, d = dom
+    achieved = d["flops"] / (d["ms"] * 1e-3) / 1e12 if d["ms"] > 0 else 0.0
+    roofline = {"kernel": name, "bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
+                "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                "avg_launch_ms": d["ms"] / max(d["launches"], 1), "launches": d["launches"],
+                "algorithmic_flop_per_launch": d["flops"] / max(d["launches"], 1)}
+    breakdown = {k: round(v["ms"], 3) for k, v in sorted(prof.items(), key=lambda kv: -kv[1]["ms"]) if v["ms"] > 0}
+    cpu = None
+    if args.cpu_iters > 0:
+        cpu = cpu_baseline(probs[0], mean_iters, args.cpu_iters, args.cpu_threads)
+    out = {"metric": METRIC, "value": value, "unit": "points/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": dt * 1e3 / args.steps, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"pnFAM GT- K=0,1, synthetic HFB Z=60 N=92 beta2={args.beta} N_sh={args.nsh} "
+                                  f"{args.ngrid}x{args.ngrid} grid, {len(w)} contour points per K per step, "
+                                  f"FP64 path, tol 1e-7, Broyden M=50",
+                      "dqp": int(basis.dqp), "nxy_K0": engines[0].nxy, "nxy_K1": engines[1].nxy,
+                      "points_per_step": len(engines) * len(w)},
+           "points_converged": conv_total, "points_total": total_pts, "mean_iterations": mean_iters,
+           "roofline": roofline, "kernel_ms": breakdown, "cpu_baseline": cpu}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

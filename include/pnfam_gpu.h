@@ -114,6 +114,18 @@ int pnfam_gpu_sp2qp(pnfam_gpu_ctx *h, const double *hsp, double *h20_re, double 
  * quadrature weights are stripped (pnfam_hamiltonian_blas.f90:679-709). */
 int pnfam_gpu_densities(pnfam_gpu_ctx *h, const double *in, int32_t sector, double *dens);
 
+/* ---- measurement (no counterpart in the reference) ---------------------- */
+
+/* Resets the per-kernel totals and turns HIP-event timing of every launch on
+ * the engine stream on (enable != 0) or off. */
+int pnfam_gpu_profile(pnfam_gpu_ctx *h, int32_t enable);
+
+/* Per kernel: name (32 bytes each), total milliseconds, launches, processed
+ * frequency-units (sum of active frequencies over launches) and the
+ * algorithmic FLOP per frequency-unit of that kernel (0 for data movement). */
+int pnfam_gpu_profile_read(pnfam_gpu_ctx *h, int32_t max_kernels, char *names, double *ms, int64_t *launches,
+                           double *units, double *flops_per_unit, int32_t *n_out);
+
 void pnfam_gpu_destroy(pnfam_gpu_ctx *h);
 const char *pnfam_gpu_last_error(void);
 

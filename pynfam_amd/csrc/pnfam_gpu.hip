@@ -814,10 +814,65 @@ struct pnfam_gpu_ctx {
   DevPtr d_vin, d_vout, d_rsp, d_hsp, d_hqp, d_dens, d_flds, d_part, d_cpart, d_S, d_si, d_scal;
   DevPtr d_df, d_dv, d_gram, d_coef, d_wre, d_wim, d_act;
   int bro_nwg = 0;
-  ~pnfam_gpu_ctx() { if (stream) (void)hipStreamDestroy(stream); }
+  double flops_q2s = 0, flops_s2q = 0;
+  // per-kernel event timing (pnfam_gpu_profile)
+  bool prof = false;
+  struct KStat { double ms = 0; int64_t launches = 0; double units = 0; };
+  KStat kstat[16];
+  struct Pending { int kid; hipEvent_t a, b; int units; };
+  std::vector<Pending> pending;
+  std::vector<hipEvent_t> event_pool;
+  double flops_unit[16] = {0};   // algorithmic FLOP per frequency, per kernel id
+  ~pnfam_gpu_ctx() {
+    for (auto &p : pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
+    for (auto e : event_pool) (void)hipEventDestroy(e);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
 };
 
 namespace {
+
+enum KernelId { K_TRIPROD_Q2S, K_DENSITY, K_FIELDS, K_PROJECT, K_TRIPROD_S2Q, K_GREEN, K_BRO_RESID, K_BRO_NORM,
+                K_BRO_DOTS, K_BRO_SOLVE, K_BRO_UPDATE, K_CONTRACT, K_CONTRACT_FIN, K_ZERO, K_NKERNEL };
+const char *kKernelNames[K_NKERNEL] = {"triprod_qp2sp", "density", "fields", "project", "triprod_sp2qp", "green",
+                                       "bro_resid", "bro_norm", "bro_dots", "bro_solve", "bro_update", "contract",
+                                       "contract_fin", "zero_slots"};
+
+hipEvent_t pool_event(pnfam_gpu_ctx *h) {
+  if (!h->event_pool.empty()) { hipEvent_t e = h->event_pool.back(); h->event_pool.pop_back(); return e; }
+  hipEvent_t e = nullptr;
+  (void)hipEventCreate(&e);
+  return e;
+}
+
+// Brackets one launch with events on the engine stream when profiling is on.
+struct Prof {
+  pnfam_gpu_ctx *h; int kid; int units; hipEvent_t a = nullptr;
+  Prof(pnfam_gpu_ctx *h_, int kid_, int units_) : h(h_), kid(kid_), units(units_) {
+    if (h->prof) { a = pool_event(h); (void)hipEventRecord(a, h->stream); }
+  }
+  ~Prof() {
+    if (!h->prof) return;
+    hipEvent_t b = pool_event(h);
+    (void)hipEventRecord(b, h->stream);
+    h->pending.push_back({kid, a, b, units});
+  }
+};
+
+// Call after a stream synchronisation: folds finished launches into the totals.
+void prof_collect(pnfam_gpu_ctx *h) {
+  for (auto &p : h->pending) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+      h->kstat[p.kid].ms += ms;
+      h->kstat[p.kid].launches += 1;
+      h->kstat[p.kid].units += p.units;
+    }
+    h->event_pool.push_back(p.a);
+    h->event_pool.push_back(p.b);
+  }
+  h->pending.clear();
+}
 
 int upload(DevPtr &d, const void *h, size_t bytes, hipStream_t s) {
   if (int e = d.alloc(bytes)) return e;
@@ -871,7 +926,9 @@ dim3 grid1(size_t n, int bs, int maxb = 2048) {
 // ---- pipeline stages over the active list (device pointer act, count na) --
 int stage_qp2sp(pnfam_gpu_ctx *h, const double *vin, int na, const int *act) {
   const long nxy = h->nxy;
-  hipLaunchKernelGGL(k_zero_slots, dim3(64, na), dim3(256), 0, h->stream, h->d_rsp.as<double>(), act, 8, 0, 8, nxy);
+  { Prof pr(h, K_ZERO, na);
+    hipLaunchKernelGGL(k_zero_slots, dim3(64, na), dim3(256), 0, h->stream, h->d_rsp.as<double>(), act, 8, 0, 8, nxy); }
+  Prof pr(h, K_TRIPROD_Q2S, na);
   for (int ri = 0; ri < 2; ++ri) {
     if (!h->ngroups_q2s[ri]) continue;
     hipLaunchKernelGGL(k_triprod, dim3(h->ngroups_q2s[ri], na), dim3(256), h->lds_q2s, h->stream,
@@ -897,7 +954,8 @@ int stage_hamiltonian(pnfam_gpu_ctx *h, int na, const int *act) {
   // sector 0: rho_pn = m11 (slot 0), kappa+ = m12 (slot 1); sector 1: rho_np = m22 (3), kappa- = m21 (2)
   A.rho_slot[0] = 0; A.kap_slot[0] = 1; A.rho_slot[1] = 3; A.kap_slot[1] = 2;
   A.dens = h->d_dens.as<double>(); A.act = act;
-  hipLaunchKernelGGL(k_density, dim3((h->nghl + 63) / 64, na, 2), dim3(64), 0, h->stream, A);
+  { Prof pr(h, K_DENSITY, na);
+    hipLaunchKernelGGL(k_density, dim3((h->nghl + 63) / 64, na, 2), dim3(64), 0, h->stream, A); }
   FieldArgs F{};
   F.dens = h->d_dens.as<double>(); F.flds = h->d_flds.as<double>();
   F.crho = h->d_crho.as<double>(); F.cs = h->d_cs.as<double>(); F.cpair = h->d_cpair.as<double>();
@@ -906,22 +964,29 @@ int stage_hamiltonian(pnfam_gpu_ctx *h, int na, const int *act) {
   F.cdrho = c[0]; F.ctau = c[1]; F.ctj0 = c[2]; F.ctj1 = c[3]; F.ctj2 = c[4]; F.crdj = c[5];
   F.cds = c[6]; F.ct = c[7]; F.cj = c[8]; F.cgs = c[9]; F.cf = c[10]; F.csdj = c[11];
   F.nghl = h->nghl; F.act = act;
-  hipLaunchKernelGGL(k_fields, dim3((h->nghl + 127) / 128, na, 2), dim3(128), 0, h->stream, F);
-  hipLaunchKernelGGL(k_zero_slots, dim3(64, na), dim3(256), 0, h->stream, h->d_hsp.as<double>(), act, 8, 0, 8, h->nxy);
+  { Prof pr(h, K_FIELDS, na);
+    hipLaunchKernelGGL(k_fields, dim3((h->nghl + 127) / 128, na, 2), dim3(128), 0, h->stream, F); }
+  { Prof pr(h, K_ZERO, na);
+    hipLaunchKernelGGL(k_zero_slots, dim3(64, na), dim3(256), 0, h->stream, h->d_hsp.as<double>(), act, 8, 0, 8, h->nxy); }
   ProjArgs P{};
   P.psi[0] = h->d_wf.as<double>(); P.psi[1] = h->d_wfdr.as<double>(); P.psi[2] = h->d_wfdp.as<double>();
   P.psi[3] = h->d_wfdz.as<double>(); P.psi[4] = h->d_wfd2.as<double>();
   P.isstart = h->d_isstart.as<int>(); P.db = h->d_db.as<int>(); P.nsu = h->d_nsu.as<int>(); P.ns = h->d_ns.as<int>();
   P.flds = h->d_flds.as<double>(); P.hsp = h->d_hsp.as<double>(); P.nxy = h->nxy; P.nghl = h->nghl;
   P.tiles = h->d_tiles.as<Tile>(); P.act = act;
-  if (h->ntiles) hipLaunchKernelGGL(k_project, dim3(h->ntiles, na), dim3(256), 0, h->stream, P);
+  if (h->ntiles) {
+    Prof pr(h, K_PROJECT, na);
+    hipLaunchKernelGGL(k_project, dim3(h->ntiles, na), dim3(256), 0, h->stream, P);
+  }
   HIP_TRY(hipGetLastError());
   return 0;
 }
 
 int stage_sp2qp(pnfam_gpu_ctx *h, int na, const int *act) {
   const long nxy = h->nxy;
-  hipLaunchKernelGGL(k_zero_slots, dim3(64, na), dim3(256), 0, h->stream, h->d_hqp.as<double>(), act, 4, 0, 4, nxy);
+  { Prof pr(h, K_ZERO, na);
+    hipLaunchKernelGGL(k_zero_slots, dim3(64, na), dim3(256), 0, h->stream, h->d_hqp.as<double>(), act, 4, 0, 4, nxy); }
+  Prof pr(h, K_TRIPROD_S2Q, na);
   for (int ri = 0; ri < 2; ++ri) {
     if (!h->ngroups_s2q[ri]) continue;
     hipLaunchKernelGGL(k_triprod, dim3(h->ngroups_s2q[ri], na), dim3(256), h->lds_s2q, h->stream,
@@ -1068,6 +1133,8 @@ int pnfam_gpu_set_operator(pnfam_gpu_ctx *h, int32_t beta_minus, const int32_t *
     size_t lds = 0;
     for (auto &g : groups) lds = std::max(lds, (size_t)g.nrow * g.ncol * 8);
     h->lds_q2s = std::max(h->lds_q2s, lds);
+    if (ri == 0) h->flops_q2s = 0;
+    for (auto &t : terms) h->flops_q2s += 2.0 * t.nda * t.nda * t.ndc + 2.0 * t.nda * t.ndc * t.ndc;
     int e;
     if ((e = upload(h->d_terms_q2s[ri], terms.data(), std::max<size_t>(1, terms.size()) * sizeof(Term), h->stream)) ||
         (e = upload(h->d_groups_q2s[ri], groups.data(), std::max<size_t>(1, groups.size()) * sizeof(Group), h->stream)))
@@ -1100,6 +1167,8 @@ int pnfam_gpu_set_operator(pnfam_gpu_ctx *h, int32_t beta_minus, const int32_t *
     size_t lds = 0;
     for (auto &g : groups) lds = std::max(lds, (size_t)g.nrow * g.ncol * 8);
     h->lds_s2q = std::max(h->lds_s2q, lds);
+    if (ri == 0) h->flops_s2q = 0;
+    for (auto &t : terms) h->flops_s2q += 2.0 * t.nda * t.nda * t.ndc + 2.0 * t.nda * t.ndc * t.ndc;
     int e;
     if ((e = upload(h->d_terms_s2q[ri], terms.data(), std::max<size_t>(1, terms.size()) * sizeof(Term), h->stream)) ||
         (e = upload(h->d_groups_s2q[ri], groups.data(), std::max<size_t>(1, groups.size()) * sizeof(Group), h->stream)))
@@ -1187,10 +1256,51 @@ int pnfam_gpu_set_operator(pnfam_gpu_ctx *h, int32_t beta_minus, const int32_t *
     HIP_TRY(hipMemcpyAsync(h->d_f02.p, dout.as<double>() + nxy, nxy * 8, hipMemcpyDeviceToDevice, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
   }
+  // algorithmic FLOP per frequency and iteration for the kernels that carry the arithmetic
+  for (double &f : h->flops_unit) f = 0.0;
+  {
+    const double G = h->nghl;
+    double q2s = 0, s2q = 0;
+    // recompute from the term tables kept on the host side of the planner
+    q2s = h->flops_q2s; s2q = h->flops_s2q;
+    const double nr0 = (double)h->rspmap[0].nelem(db), nk0 = (double)h->rspmap[1].nelem(db);
+    const double nr1 = (double)h->rspmap[3].nelem(db), nk1 = (double)h->rspmap[2].nelem(db);
+    h->flops_unit[K_TRIPROD_Q2S] = q2s;
+    h->flops_unit[K_TRIPROD_S2Q] = s2q;
+    h->flops_unit[K_DENSITY] = G * (16.0 * (nr0 + nr1) + 4.0 * (nk0 + nk1));
+    h->flops_unit[K_PROJECT] = G * (20.0 * (double)(h->hspmap[0].nelem(db) + h->hspmap[3].nelem(db)) +
+                                    4.0 * (double)(h->hspmap[1].nelem(db) + h->hspmap[2].nelem(db)));
+  }
   h->have_op = true;
   h->cap = 0;
   h->hist_cap = -1;
   if (nxy_out) *nxy_out = nxy;
+  return 0;
+}
+
+int pnfam_gpu_profile(pnfam_gpu_ctx *h, int32_t enable) {
+  if (!h) return fail(PNFAM_GPU_EINVAL, "null context");
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  prof_collect(h);
+  for (auto &k : h->kstat) k = pnfam_gpu_ctx::KStat();
+  h->prof = enable != 0;
+  return 0;
+}
+
+int pnfam_gpu_profile_read(pnfam_gpu_ctx *h, int32_t max_kernels, char *names, double *ms, int64_t *launches,
+                           double *units, double *flops_per_unit, int32_t *n_out) {
+  if (!h || !n_out) return fail(PNFAM_GPU_EINVAL, "null argument");
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  prof_collect(h);
+  const int n = std::min<int>(max_kernels, K_NKERNEL);
+  for (int k = 0; k < n; ++k) {
+    if (names) { std::memset(names + 32 * k, 0, 32); std::strncpy(names + 32 * k, kKernelNames[k], 31); }
+    if (ms) ms[k] = h->kstat[k].ms;
+    if (launches) launches[k] = h->kstat[k].launches;
+    if (units) units[k] = h->kstat[k].units;
+    if (flops_per_unit) flops_per_unit[k] = h->flops_unit[k];
+  }
+  *n_out = n;
   return 0;
 }
 
@@ -1335,7 +1445,8 @@ int pnfam_gpu_solve_batch(pnfam_gpu_ctx *h, int32_t npts, const double *re_w, co
     G.xrow = h->d_xrow.as<int>(); G.xcol = h->d_xcol.as<int>(); G.yrow = h->d_yrow.as<int>(); G.ycol = h->d_ycol.as<int>();
     G.wre = h->d_wre.as<double>(); G.wim = h->d_wim.as<double>(); G.vout = h->d_vout.as<double>();
     G.nxy = nxy; G.quench = o->quench; G.use_h = !skip_res; G.act = dact;
-    hipLaunchKernelGGL(k_green, dim3((unsigned)std::min<long>(1024, (2 * nxy + 255) / 256), na), dim3(256), 0, st, G);
+    { Prof pr(h, K_GREEN, na);
+      hipLaunchKernelGGL(k_green, dim3((unsigned)std::min<long>(1024, (2 * nxy + 255) / 256), na), dim3(256), 0, st, G); }
     // Broyden
     BroArgs B{};
     B.vin = h->d_vin.as<double>(); B.vout = h->d_vout.as<double>(); B.df = h->d_df.as<double>();
@@ -1352,21 +1463,24 @@ int pnfam_gpu_solve_batch(pnfam_gpu_ctx *h, int32_t npts, const double *re_w, co
       if (iter < 2) B.ipos = 0;
     }
     B.iter = mode == 0 ? iter : 0;
-    hipLaunchKernelGGL(k_bro_resid, dim3(nwg, na), dim3(BRO_WG), 0, st, B);
-    hipLaunchKernelGGL(k_bro_norm, dim3(na), dim3(64), 0, st, B);
+    { Prof pr(h, K_BRO_RESID, na); hipLaunchKernelGGL(k_bro_resid, dim3(nwg, na), dim3(BRO_WG), 0, st, B); }
+    { Prof pr(h, K_BRO_NORM, na); hipLaunchKernelGGL(k_bro_norm, dim3(na), dim3(64), 0, st, B); }
     if (mode == 0) {
-      hipLaunchKernelGGL(k_bro_dots, dim3(nwg, na), dim3(BRO_WG), 0, st, B);
-      hipLaunchKernelGGL(k_bro_solve, dim3(na), dim3(256), 0, st, B);
+      { Prof pr(h, K_BRO_DOTS, na); hipLaunchKernelGGL(k_bro_dots, dim3(nwg, na), dim3(BRO_WG), 0, st, B); }
+      { Prof pr(h, K_BRO_SOLVE, na); hipLaunchKernelGGL(k_bro_solve, dim3(na), dim3(256), 0, st, B); }
     }
-    hipLaunchKernelGGL(k_bro_update, dim3(nwg, na), dim3(BRO_WG), 0, st, B, mode);
+    { Prof pr(h, K_BRO_UPDATE, na); hipLaunchKernelGGL(k_bro_update, dim3(nwg, na), dim3(BRO_WG), 0, st, B, mode); }
     const int cnwg = 64;
-    hipLaunchKernelGGL(k_contract, dim3(cnwg, na), dim3(256), 0, st, h->d_vin.as<double>(), h->d_f20.as<double>(),
-                       h->d_f02.as<double>(), nxy, h->d_cpart.as<double>(), cnwg, dact);
-    hipLaunchKernelGGL(k_contract_fin, dim3(na), dim3(64), 0, st, h->d_cpart.as<double>(), cnwg, h->d_S.as<double>(), dact);
+    { Prof pr(h, K_CONTRACT, na);
+      hipLaunchKernelGGL(k_contract, dim3(cnwg, na), dim3(256), 0, st, h->d_vin.as<double>(), h->d_f20.as<double>(),
+                         h->d_f02.as<double>(), nxy, h->d_cpart.as<double>(), cnwg, dact); }
+    { Prof pr(h, K_CONTRACT_FIN, na);
+      hipLaunchKernelGGL(k_contract_fin, dim3(na), dim3(64), 0, st, h->d_cpart.as<double>(), cnwg, h->d_S.as<double>(), dact); }
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(si.data(), h->d_si.p, npts * 8, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipMemcpyAsync(S.data(), h->d_S.p, 2 * npts * 8, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
+    if (h->prof) prof_collect(h);
     for (int w : act) { re_s[w] = S[2 * w]; im_s[w] = S[2 * w + 1]; }
   }
   if (si_final) for (int w = 0; w < npts; ++w) si_final[w] = si[w];

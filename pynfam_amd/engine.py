@@ -67,13 +67,16 @@ def load_library(path=LIB_PATH):
     lib.pnfam_gpu_calc_hamiltonian.argtypes = [P, _dp, _dp]
     lib.pnfam_gpu_sp2qp.argtypes = [P, _dp, _dp, _dp, _dp, _dp]
     lib.pnfam_gpu_densities.argtypes = [P, _dp, C.c_int32, _dp]
+    lib.pnfam_gpu_profile.argtypes = [P, C.c_int32]
+    lib.pnfam_gpu_profile_read.argtypes = [P, C.c_int32, C.c_char_p, _dp, C.POINTER(C.c_int64), _dp, _dp,
+                                           C.POINTER(C.c_int32)]
     lib.pnfam_gpu_destroy.argtypes = [P]
     lib.pnfam_gpu_destroy.restype = None
     lib.pnfam_gpu_last_error.restype = C.c_char_p
     lib.pnfam_gpu_version.restype = C.c_char_p
     for name in ("pnfam_gpu_create", "pnfam_gpu_set_operator", "pnfam_gpu_solve_batch", "pnfam_gpu_structures",
                  "pnfam_gpu_get_fqp", "pnfam_gpu_qp2sp", "pnfam_gpu_calc_hamiltonian", "pnfam_gpu_sp2qp",
-                 "pnfam_gpu_densities"):
+                 "pnfam_gpu_densities", "pnfam_gpu_profile", "pnfam_gpu_profile_read"):
         getattr(lib, name).restype = C.c_int
     _lib = lib
     return lib
@@ -153,6 +156,24 @@ class GpuEngine:
         out = {"strength": rs + 1j * is_, "iterations": it, "si": si}
         if want_x:
             out["x"] = x
+        return out
+
+    # ---- per-kernel timing ---------------------------------------------------
+    def profile(self, enable=True):
+        self._check(self.lib.pnfam_gpu_profile(self.h, int(enable)))
+
+    def profile_read(self):
+        n = 32
+        names = C.create_string_buffer(32 * n)
+        ms, units, fpu = np.zeros(n), np.zeros(n), np.zeros(n)
+        launches = np.zeros(n, dtype=np.int64)
+        cnt = C.c_int32()
+        self._check(self.lib.pnfam_gpu_profile_read(self.h, n, names, _d(ms), launches.ctypes.data_as(
+            C.POINTER(C.c_int64)), _d(units), _d(fpu), C.byref(cnt)))
+        out = {}
+        for k in range(cnt.value):
+            nm = names.raw[32 * k:32 * (k + 1)].split(b"\0")[0].decode()
+            out[nm] = {"ms": ms[k], "launches": int(launches[k]), "units": units[k], "flops_per_unit": fpu[k]}
         return out
 
     # ---- per-unit entry points ---------------------------------------------

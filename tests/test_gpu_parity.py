@@ -155,12 +155,14 @@ def test_solve_matches_oracle_and_edge_cases(engines):
     o = orc.ifam(w[0], broyden_history=0, max_iter=40)
     assert r["iterations"][0] == o["iterations"]
     assert abs(r["strength"][0] - o["strength"]) < 1e-7 * abs(o["strength"])
-    # short Broyden history wraps around the circular buffer
-    r = eng.solve(w[:2], broyden_history=3)
-    for k in range(2):
-        o = orc.ifam(w[k], broyden_history=3)
-        assert r["iterations"][k] == o["iterations"]
-        assert abs(r["strength"][k] - o["strength"]) < 1e-7 * abs(o["strength"])
+    # short Broyden histories wrap around the circular buffer many times
+    # (w[0] converges in 24-34 steps with M = 1..3; with M <= 5 the modified
+    # Broyden iteration itself diverges at w[1], in the oracle as well)
+    for M in (1, 2, 3):
+        r = eng.solve(w[:1], broyden_history=M)
+        o = orc.ifam(w[0], broyden_history=M)
+        assert r["iterations"][0] == o["iterations"] > 3 * M
+        assert abs(r["strength"][0] - o["strength"]) < 1e-7 * abs(o["strength"])
 
 
 def test_beta_plus():
