@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -290,6 +291,227 @@ __global__ void __launch_bounds__(64) k_density(DensArgs A) {
   for (int d = 0; d < 2 * NDENS; ++d) out[(size_t)d * G] = acc[d] * wc;
 }
 
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+// v_mfma_f64_16x16x4: lane i + 16k supplies A[i][k] and B[k][i]; the result
+// D[i][j] is held by lane j + 16 (i % 4) in register i / 4.
+__device__ inline v4d mfma64(double a, double b, v4d c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// Accumulates the local-density contributions of one column b at one grid
+// point from T_x^s (x = psi, d_r psi, d_phi psi, d_z psi; s = spin of a) and
+// psi_y(b) (pnfam_hamiltonian_blas.f90:255-581).
+__device__ inline void density_column(double *acc, const double (&Tr)[4][2], const double (&Ti)[4][2],
+                                      const double (&y)[4], int s) {
+#define ACC(d, ri) acc[2 * (d) + (ri)]
+  const int sd = s > 0 ? 0 : 1, so = 1 - sd;
+  double Pr[4][4], Pi[4][4], Qr[4][4], Qi[4][4];
+#pragma unroll
+  for (int x = 0; x < 4; ++x)
+#pragma unroll
+    for (int yy = 0; yy < 4; ++yy) {
+      Pr[x][yy] = Tr[x][sd] * y[yy]; Pi[x][yy] = Ti[x][sd] * y[yy];
+      Qr[x][yy] = Tr[x][so] * y[yy]; Qi[x][yy] = Ti[x][so] * y[yy];
+    }
+  double ar, ai;
+  ACC(D_RHO, 0) += Pr[0][0]; ACC(D_RHO, 1) += Pi[0][0];
+  ACC(D_SZ, 0) += s * Pr[0][0]; ACC(D_SZ, 1) += s * Pi[0][0];
+  ar = 0.5 * (Pr[0][2] + Pr[2][0]); ai = 0.5 * (Pi[0][2] + Pi[2][0]);
+  ACC(D_JP, 0) += ar; ACC(D_JP, 1) += ai;
+  ACC(D_TJPZ, 0) += s * ar; ACC(D_TJPZ, 1) += s * ai;
+  ar = Pr[1][1] + Pr[2][2] + Pr[3][3]; ai = Pi[1][1] + Pi[2][2] + Pi[3][3];
+  ACC(D_TAU, 0) += ar; ACC(D_TAU, 1) += ai;
+  ACC(D_TZ, 0) += s * ar; ACC(D_TZ, 1) += s * ai;
+  ar = 0.5 * (Pr[0][1] - Pr[1][0]); ai = 0.5 * (Pi[0][1] - Pi[1][0]);
+  ACC(D_JR, 0) -= ai; ACC(D_JR, 1) += ar;
+  ACC(D_TJRZ, 0) -= s * ai; ACC(D_TJRZ, 1) += s * ar;
+  ar = 0.5 * (Pr[0][3] - Pr[3][0]); ai = 0.5 * (Pi[0][3] - Pi[3][0]);
+  ACC(D_JZ, 0) -= ai; ACC(D_JZ, 1) += ar;
+  ACC(D_TJZZ, 0) -= s * ai; ACC(D_TJZZ, 1) += s * ar;
+  ACC(D_GS, 0) += s * (Pr[0][3] + Pr[3][0]); ACC(D_GS, 1) += s * (Pi[0][3] + Pi[3][0]);
+  ACC(D_FR, 0) += 0.5 * s * (Pr[1][3] + Pr[3][1]); ACC(D_FR, 1) += 0.5 * s * (Pi[1][3] + Pi[3][1]);
+  ar = Pr[2][3] - Pr[3][2]; ai = Pi[2][3] - Pi[3][2];
+  ACC(D_FP, 0) -= 0.5 * s * ai; ACC(D_FP, 1) += 0.5 * s * ar;
+  ACC(D_FZ, 0) += s * Pr[3][3]; ACC(D_FZ, 1) += s * Pi[3][3];
+  const double e = -(double)s;
+  ACC(D_SR, 0) += Qr[0][0]; ACC(D_SR, 1) += Qi[0][0];
+  ACC(D_SP, 0) -= e * Qi[0][0]; ACC(D_SP, 1) += e * Qr[0][0];
+  const double t2r = 0.5 * (Qr[2][0] + Qr[0][2]), t2i = 0.5 * (Qi[2][0] + Qi[0][2]);
+  ACC(D_TJPR, 0) += t2r; ACC(D_TJPR, 1) += t2i;
+  ACC(D_TJPP, 0) -= e * t2i; ACC(D_TJPP, 1) += e * t2r;
+  ACC(D_GS, 0) += e * (Qr[0][2] - Qr[2][0]) + (Qr[0][1] + Qr[1][0]);
+  ACC(D_GS, 1) += e * (Qi[0][2] - Qi[2][0]) + (Qi[0][1] + Qi[1][0]);
+  const double tkr = Qr[1][1] + Qr[2][2] + Qr[3][3], tki = Qi[1][1] + Qi[2][2] + Qi[3][3];
+  ACC(D_TR, 0) += tkr; ACC(D_TR, 1) += tki;
+  ACC(D_TP, 0) -= e * tki; ACC(D_TP, 1) += e * tkr;
+  const double jrr = 0.5 * (Qr[0][1] - Qr[1][0]), jri = 0.5 * (Qi[0][1] - Qi[1][0]);
+  ACC(D_TJRR, 0) -= jri; ACC(D_TJRR, 1) += jrr;
+  ACC(D_TJRP, 0) -= e * jrr; ACC(D_TJRP, 1) -= e * jri;
+  const double jzr = 0.5 * (Qr[0][3] - Qr[3][0]), jzi = 0.5 * (Qi[0][3] - Qi[3][0]);
+  ACC(D_TJZR, 0) -= jzi; ACC(D_TJZR, 1) += jzr;
+  ACC(D_TJZP, 0) -= e * jzr; ACC(D_TJZP, 1) -= e * jzi;
+  ar = Qr[1][1] + e * 0.5 * (Qr[1][2] - Qr[2][1]);
+  ai = Qi[1][1] + e * 0.5 * (Qi[1][2] - Qi[2][1]);
+  ACC(D_FR, 0) += ar; ACC(D_FR, 1) += ai;
+  ar = e * Qr[2][2] + 0.5 * (Qr[2][1] - Qr[1][2]);
+  ai = e * Qi[2][2] + 0.5 * (Qi[2][1] - Qi[1][2]);
+  ACC(D_FP, 0) -= ai; ACC(D_FP, 1) += ar;
+  ar = Qr[3][1] + Qr[1][3] + e * (Qr[3][2] - Qr[2][3]);
+  ai = Qi[3][1] + Qi[1][3] + e * (Qi[3][2] - Qi[2][3]);
+  ACC(D_FZ, 0) += 0.5 * ar; ACC(D_FZ, 1) += 0.5 * ai;
+#undef ACC
+}
+
+// Local densities with FP64 MFMA (v_mfma_f64_16x16x4).  One wave per
+// (16 grid points, frequency, sector), looping over all block pairs.
+// T_x^s(r, b) = sum_a psi_x(r,a) rho(a,b) runs on the matrix cores (rows r,
+// k = a in steps of 4, 16 columns b per tile, re and im tiles), per-spin ranges
+// of a accumulated separately; the tile goes through LDS to a lane-per-(r, 4 b)
+// layout for the density products, whose sums over b stay in registers.
+__global__ void __launch_bounds__(64) k_density_mfma(DensArgs A) {
+  __shared__ double sT[16][16][17];     // [(x, s, ri)][r][b]
+  const int lane = threadIdx.x;
+  const int r0 = blockIdx.x * 16;
+  const int w = A.act[blockIdx.y];
+  const int sec = blockIdx.z;
+  const int G = A.nghl;
+  const int mi = lane & 15, mk = lane >> 4;     // MFMA operand lane coordinates
+  const int er = lane & 15, eg = lane >> 4;     // epilogue: row er, columns eg + 4q
+  const int rA = r0 + mi;
+  const bool rA_ok = rA < G;
+  const int rE = r0 + er;
+  const bool rE_ok = rE < G;
+  const int rEc = rE_ok ? rE : 0;
+  double acc[2 * NDENS];
+#pragma unroll
+  for (int d = 0; d < 2 * NDENS; ++d) acc[d] = 0.0;
+  const double *rho_re = A.rsp + ((size_t)w * 8 + 2 * A.rho_slot[sec]) * A.nxy;
+  const double *rho_im = rho_re + A.nxy;
+  const double *tab[4] = {A.wf, A.wfdr, A.wfdp, A.wfdz};
+  const PairList pl = A.rho[sec];
+  for (int p = 0; p < pl.n; ++p) {
+    const int4 pr = pl.p[p];
+    const int ix = pr.x, iy = pr.y, off = pr.z;
+    const int ia = A.isstart[ix], ib0 = A.isstart[iy], dx = A.db[ix], dy = A.db[iy], nsu = A.nsu[ix];
+    for (int b0 = 0; b0 < dy; b0 += 16) {
+      v4d C[4][2][2];
+#pragma unroll
+      for (int x = 0; x < 4; ++x)
+#pragma unroll
+        for (int s2 = 0; s2 < 2; ++s2) { C[x][s2][0] = v4d{0, 0, 0, 0}; C[x][s2][1] = v4d{0, 0, 0, 0}; }
+      const int bj = b0 + mi;
+#pragma unroll
+      for (int s2 = 0; s2 < 2; ++s2) {
+        const int lo = s2 ? nsu : 0, hi = s2 ? dx : nsu;
+        for (int a = lo; a < hi; a += 4) {
+          const int ak = a + mk;
+          const bool ok = ak < hi;
+          const size_t oa = (size_t)(ia + (ok ? ak : lo)) * G + (rA_ok ? rA : 0);
+          double av[4];
+#pragma unroll
+          for (int x = 0; x < 4; ++x) av[x] = (ok && rA_ok) ? tab[x][oa] : 0.0;
+          const bool bok = ok && bj < dy;
+          const size_t ob = off + (size_t)(bok ? bj : 0) * dx + (ok ? ak : 0);
+          const double br = bok ? rho_re[ob] : 0.0, bi = bok ? rho_im[ob] : 0.0;
+#pragma unroll
+          for (int x = 0; x < 4; ++x) {
+            C[x][s2][0] = mfma64(av[x], br, C[x][s2][0]);
+            C[x][s2][1] = mfma64(av[x], bi, C[x][s2][1]);
+          }
+        }
+      }
+#pragma unroll
+      for (int x = 0; x < 4; ++x)
+#pragma unroll
+        for (int s2 = 0; s2 < 2; ++s2)
+#pragma unroll
+          for (int ri = 0; ri < 2; ++ri)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) sT[(x * 2 + s2) * 2 + ri][4 * v + mk][mi] = C[x][s2][ri][v];
+      __syncthreads();
+      for (int q = 0; q < 4; ++q) {
+        const int bl = eg + 4 * q;
+        if (b0 + bl >= dy) break;
+        const int b = ib0 + b0 + bl;
+        double Tr[4][2], Ti[4][2];
+#pragma unroll
+        for (int x = 0; x < 4; ++x)
+#pragma unroll
+          for (int s2 = 0; s2 < 2; ++s2) {
+            Tr[x][s2] = sT[(x * 2 + s2) * 2 + 0][er][bl];
+            Ti[x][s2] = sT[(x * 2 + s2) * 2 + 1][er][bl];
+          }
+        const size_t oy = (size_t)b * G + rEc;
+        const double y[4] = {A.wf[oy], A.wfdr[oy], A.wfdp[oy], A.wfdz[oy]};
+        density_column(acc, Tr, Ti, y, A.ns[b]);
+      }
+      __syncthreads();
+    }
+  }
+  // pairing densities from kappa: x = 0 only
+  const double *kap_re = A.rsp + ((size_t)w * 8 + 2 * A.kap_slot[sec]) * A.nxy;
+  const double *kap_im = kap_re + A.nxy;
+  const PairList kl = A.kap[sec];
+  for (int p = 0; p < kl.n; ++p) {
+    const int4 pr = kl.p[p];
+    const int ix = pr.x, iy = pr.y, off = pr.z;
+    const int ia = A.isstart[ix], ib0 = A.isstart[iy], dx = A.db[ix], dy = A.db[iy], nsu = A.nsu[ix];
+    for (int b0 = 0; b0 < dy; b0 += 16) {
+      v4d C[2][2];
+#pragma unroll
+      for (int s2 = 0; s2 < 2; ++s2) { C[s2][0] = v4d{0, 0, 0, 0}; C[s2][1] = v4d{0, 0, 0, 0}; }
+      const int bj = b0 + mi;
+#pragma unroll
+      for (int s2 = 0; s2 < 2; ++s2) {
+        const int lo = s2 ? nsu : 0, hi = s2 ? dx : nsu;
+        for (int a = lo; a < hi; a += 4) {
+          const int ak = a + mk;
+          const bool ok = ak < hi;
+          const double av = (ok && rA_ok) ? A.wf[(size_t)(ia + ak) * G + rA] : 0.0;
+          const bool bok = ok && bj < dy;
+          const size_t ob = off + (size_t)(bok ? bj : 0) * dx + (ok ? ak : 0);
+          const double br = bok ? kap_re[ob] : 0.0, bi = bok ? kap_im[ob] : 0.0;
+          C[s2][0] = mfma64(av, br, C[s2][0]);
+          C[s2][1] = mfma64(av, bi, C[s2][1]);
+        }
+      }
+#pragma unroll
+      for (int s2 = 0; s2 < 2; ++s2)
+#pragma unroll
+        for (int ri = 0; ri < 2; ++ri)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) sT[s2 * 2 + ri][4 * v + mk][mi] = C[s2][ri][v];
+      __syncthreads();
+      for (int q = 0; q < 4; ++q) {
+        const int bl = eg + 4 * q;
+        if (b0 + bl >= dy) break;
+        const int b = ib0 + b0 + bl;
+        const int s = A.ns[b];
+        const int sd = s > 0 ? 0 : 1, so = 1 - sd;
+        const double wb = A.wf[(size_t)b * G + rEc];
+        double pr_ = 2.0 * sT[so * 2 + 0][er][bl] * wb, pi_ = 2.0 * sT[so * 2 + 1][er][bl] * wb;
+        acc[2 * D_RB] -= s * pr_; acc[2 * D_RB + 1] -= s * pi_;
+        acc[2 * D_SBZ] += pr_; acc[2 * D_SBZ + 1] += pi_;
+        pr_ = 2.0 * sT[sd * 2 + 0][er][bl] * wb; pi_ = 2.0 * sT[sd * 2 + 1][er][bl] * wb;
+        acc[2 * D_SBR] += s < 0 ? pr_ : -pr_; acc[2 * D_SBR + 1] += s < 0 ? pi_ : -pi_;
+        acc[2 * D_SBP] += pi_; acc[2 * D_SBP + 1] -= pr_;
+      }
+      __syncthreads();
+    }
+  }
+  // sum the four column subgroups (lanes er, er+16, er+32, er+48)
+  const double wc = rE_ok ? A.wdcori[rE] : 0.0;
+  double *out = A.dens + (((size_t)w * 2 + sec) * 2 * NDENS) * G + rEc;
+#pragma unroll
+  for (int d = 0; d < 2 * NDENS; ++d) {
+    double v = acc[d];
+    v += __shfl_xor(v, 16, 64);
+    v += __shfl_xor(v, 32, 64);
+    if (eg == 0 && rE_ok) out[(size_t)d * G] = v * wc;
+  }
+}
+
 struct FieldArgs {
   const double *dens;     // [w][2][56][nghl]
   double *flds;           // [w][2][208][nghl]
@@ -510,6 +732,114 @@ __global__ void __launch_bounds__(256) k_project(ProjArgs P) {
       const size_t o = (size_t)(t.a0 + ta) + (size_t)(t.b0 + b) * dx;
       ore[o] = 2.0 * accr[k];
       oim[o] = 2.0 * acci[k];
+    }
+  }
+}
+
+struct Tile2 {        // one wave's output of a projected matrix: rows of one spin, 16 columns
+  int32_t ix, iy, off, sec;
+  int32_t a_lo, a_hi, sa, b0;     // row range inside block ix (single spin sa), column tile origin
+  int32_t out_slot, kind, pad0, pad1;
+};
+static_assert(sizeof(Tile2) == 48, "Tile2 layout");
+
+constexpr int P2_MAXRT = 4;   // row tiles of 16 per wave
+
+// h_ab / Delta_ab with FP64 MFMA: D[a][b] = sum_{z,r} psi_z(r,a) B_z(r,b), with
+// B_z(r,b) = sum_z' mf(z,z',s_a,s_b)(r) psi_z'(r,b) built in registers from
+// fields staged in LDS per 16 grid points (kind 1: B = P_{s_a s_b}(r) psi_0(r,b)).
+// Rows: up to 64 states of spin s_a of block ix; columns: 16 states of block iy.
+__global__ void __launch_bounds__(64) k_project_mfma(ProjArgs P, const Tile2 *__restrict__ tiles) {
+  __shared__ double sMF[16][2][5][5][2];   // [r][s_b][z][z'][ri]
+  const Tile2 t = tiles[blockIdx.x];
+  const int w = P.act[blockIdx.y];
+  const int G = P.nghl;
+  const int lane = threadIdx.x;
+  const int mi = lane & 15, mk = lane >> 4;
+  const int ia = P.isstart[t.ix], ib0 = P.isstart[t.iy];
+  const int dx = P.db[t.ix], dy = P.db[t.iy];
+  const int nrows = t.a_hi - t.a_lo;
+  const int nrt = (nrows + 15) / 16;
+  const int nz = t.kind == 0 ? 5 : 1;
+  const double *fl = P.flds + (((size_t)w * 2 + t.sec) * 2 * NFLD) * G;
+  const int bj = t.b0 + mi;             // this lane's B column
+  const bool bcol = bj < dy;
+  const int sb = bcol ? (P.ns[ib0 + bj] > 0 ? 0 : 1) : 0;
+  v4d C[P2_MAXRT][2];
+#pragma unroll
+  for (int q = 0; q < P2_MAXRT; ++q) { C[q][0] = v4d{0, 0, 0, 0}; C[q][1] = v4d{0, 0, 0, 0}; }
+  for (int rc = 0; rc < G; rc += 16) {
+    // stage the fields of 16 grid points for this spin of a
+    if (t.kind == 0) {
+      for (int e = lane; e < 16 * 2 * 25 * 2; e += 64) {
+        const int ri = e & 1, zz = (e >> 1) % 25, sbb = ((e >> 1) / 25) & 1, rr = (e >> 1) / 50;
+        const int r = rc + rr;
+        const int idx = mf_index(zz / 5, zz % 5, t.sa, sbb);
+        sMF[rr][sbb][zz / 5][zz % 5][ri] = r < G ? fl[(size_t)(2 * idx + ri) * G + r] : 0.0;
+      }
+    } else {
+      for (int e = lane; e < 16 * 2 * 2; e += 64) {
+        const int ri = e & 1, sbb = (e >> 1) & 1, rr = e >> 2;
+        const int r = rc + rr;
+        const int idx = 100 + (sbb == 0 ? (t.sa == 0 ? 0 : 1) : (t.sa == 0 ? 2 : 3));
+        sMF[rr][sbb][0][0][ri] = r < G ? fl[(size_t)(2 * idx + ri) * G + r] : 0.0;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kc = 0; kc < 16; kc += 4) {
+      const int rr = kc + mk;
+      const int r = rc + rr;
+      const bool rok = r < G;
+      // B operand (k = r, column bj): B_z = sum_z' mf * psi_z'(r, b)
+      double pz[5];
+      const size_t ob = (size_t)(ib0 + (bcol ? bj : 0)) * G + (rok ? r : 0);
+#pragma unroll
+      for (int z = 0; z < 5; ++z) pz[z] = (z < nz && bcol && rok) ? P.psi[z][ob] : 0.0;
+      double Bre[5], Bim[5];
+#pragma unroll
+      for (int z = 0; z < 5; ++z) {
+        double br = 0.0, bi = 0.0;
+#pragma unroll
+        for (int zp = 0; zp < 5; ++zp) {
+          br += sMF[rr][sb][z][zp][0] * pz[zp];
+          bi += sMF[rr][sb][z][zp][1] * pz[zp];
+        }
+        Bre[z] = br;
+        Bim[z] = bi;
+      }
+      // A operand (row a, k = r) and the products
+#pragma unroll
+      for (int q = 0; q < P2_MAXRT; ++q) {
+        if (q >= nrt) break;
+        const int arow = q * 16 + mi;
+        const bool aok = arow < nrows && rok;
+        const size_t oa = (size_t)(ia + t.a_lo + (aok ? arow : 0)) * G + (rok ? r : 0);
+#pragma unroll
+        for (int z = 0; z < 5; ++z) {
+          if (z >= nz) break;
+          const double av = aok ? P.psi[z][oa] : 0.0;
+          C[q][0] = mfma64(av, Bre[z], C[q][0]);
+          C[q][1] = mfma64(av, Bim[z], C[q][1]);
+        }
+      }
+    }
+    __syncthreads();
+  }
+  double *ore = P.hsp + ((size_t)w * 8 + 2 * t.out_slot) * P.nxy + t.off;
+  double *oim = ore + P.nxy;
+  const int bcolo = t.b0 + (lane & 15);
+#pragma unroll
+  for (int q = 0; q < P2_MAXRT; ++q) {
+    if (q >= nrt) break;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int arow = q * 16 + 4 * v + (lane >> 4);
+      if (arow < nrows && bcolo < dy) {
+        const size_t o = (size_t)(t.a_lo + arow) + (size_t)bcolo * dx;
+        ore[o] = 2.0 * C[q][0][v];
+        oim[o] = 2.0 * C[q][1][v];
+      }
     }
   }
 }
@@ -806,8 +1136,9 @@ struct pnfam_gpu_ctx {
   size_t lds_q2s = 0, lds_s2q = 0;
   DevPtr d_pairs;   // rho0, kap0, rho1, kap1 pair lists
   int pair_off[4] = {0, 0, 0, 0}, pair_n[4] = {0, 0, 0, 0};
-  DevPtr d_tiles;
-  int ntiles = 0;
+  DevPtr d_tiles, d_tiles2;
+  int ntiles = 0, ntiles2 = 0;
+  bool basic_kernels = false;   // PNFAM_GPU_BASIC=1: the plain FP64 kernels (cross-check path)
   // per-frequency workspace
   int cap = 0;
   int hist_cap = -1;
@@ -955,7 +1286,10 @@ int stage_hamiltonian(pnfam_gpu_ctx *h, int na, const int *act) {
   A.rho_slot[0] = 0; A.kap_slot[0] = 1; A.rho_slot[1] = 3; A.kap_slot[1] = 2;
   A.dens = h->d_dens.as<double>(); A.act = act;
   { Prof pr(h, K_DENSITY, na);
-    hipLaunchKernelGGL(k_density, dim3((h->nghl + 63) / 64, na, 2), dim3(64), 0, h->stream, A); }
+    if (h->basic_kernels)
+      hipLaunchKernelGGL(k_density, dim3((h->nghl + 63) / 64, na, 2), dim3(64), 0, h->stream, A);
+    else
+      hipLaunchKernelGGL(k_density_mfma, dim3((h->nghl + 15) / 16, na, 2), dim3(64), 0, h->stream, A); }
   FieldArgs F{};
   F.dens = h->d_dens.as<double>(); F.flds = h->d_flds.as<double>();
   F.crho = h->d_crho.as<double>(); F.cs = h->d_cs.as<double>(); F.cpair = h->d_cpair.as<double>();
@@ -974,9 +1308,12 @@ int stage_hamiltonian(pnfam_gpu_ctx *h, int na, const int *act) {
   P.isstart = h->d_isstart.as<int>(); P.db = h->d_db.as<int>(); P.nsu = h->d_nsu.as<int>(); P.ns = h->d_ns.as<int>();
   P.flds = h->d_flds.as<double>(); P.hsp = h->d_hsp.as<double>(); P.nxy = h->nxy; P.nghl = h->nghl;
   P.tiles = h->d_tiles.as<Tile>(); P.act = act;
-  if (h->ntiles) {
+  if (h->basic_kernels && h->ntiles) {
     Prof pr(h, K_PROJECT, na);
     hipLaunchKernelGGL(k_project, dim3(h->ntiles, na), dim3(256), 0, h->stream, P);
+  } else if (h->ntiles2) {
+    Prof pr(h, K_PROJECT, na);
+    hipLaunchKernelGGL(k_project_mfma, dim3(h->ntiles2, na), dim3(64), 0, h->stream, P, h->d_tiles2.as<Tile2>());
   }
   HIP_TRY(hipGetLastError());
   return 0;
@@ -1035,6 +1372,7 @@ int pnfam_gpu_create(const pnfam_gpu_basis *b, pnfam_gpu_ctx **out) {
       if ((k < h->nsu[i]) != (h->ns[h->isstart[i] + k] > 0))
         return fail(PNFAM_GPU_EINVAL, "states must be spin-sorted within blocks");
   h->cscal.assign(b->cscal, b->cscal + 12);
+  if (const char *ev = std::getenv("PNFAM_GPU_BASIC")) h->basic_kernels = ev[0] == '1';
   HIP_TRY(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
   const size_t tab = (size_t)b->nghl * b->dqp * 8;
   size_t dmat = 0;
@@ -1204,6 +1542,26 @@ int pnfam_gpu_set_operator(pnfam_gpu_ctx *h, int32_t beta_minus, const int32_t *
   }
   h->ntiles = (int)tiles.size();
   if ((e = upload(h->d_tiles, tiles.data(), std::max<size_t>(1, tiles.size()) * sizeof(Tile), h->stream))) return e;
+  std::vector<Tile2> tiles2;
+  for (int q = 0; q < 4; ++q) {
+    for (auto &p : pair_list(h->hspmap[q])) {
+      const int dx = db[p.x], dy = db[p.y], nsu = h->nsu[p.x];
+      for (int sa = 0; sa < 2; ++sa) {
+        const int lo = sa ? nsu : 0, hi = sa ? dx : nsu;
+        for (int a0 = lo; a0 < hi; a0 += 16 * P2_MAXRT)
+          for (int b0 = 0; b0 < dy; b0 += 16) {
+            Tile2 t{};
+            t.ix = p.x; t.iy = p.y; t.off = p.z; t.sec = slot_sec[q];
+            t.a_lo = a0; t.a_hi = std::min(hi, a0 + 16 * P2_MAXRT); t.sa = sa; t.b0 = b0;
+            t.out_slot = q; t.kind = (q == 1 || q == 2) ? 1 : 0;
+            tiles2.push_back(t);
+          }
+      }
+    }
+  }
+  h->ntiles2 = (int)tiles2.size();
+  if ((e = upload(h->d_tiles2, tiles2.data(), std::max<size_t>(1, tiles2.size()) * sizeof(Tile2), h->stream)))
+    return e;
   // element -> (row state, column state) for the Green's function
   auto rc = [&](const BlockMap &m, std::vector<int> &row, std::vector<int> &col) {
     row.assign(nxy, 0); col.assign(nxy, 0);

