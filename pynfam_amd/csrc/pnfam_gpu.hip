@@ -54,6 +54,14 @@ __host__ __device__ constexpr int mf_index(int z, int zp, int sa, int sb) {  // 
   return ((z * 5 + zp) * 2 + sa) * 2 + sb;
 }
 
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+// v_mfma_f64_16x16x4: lane i + 16k supplies A[i][k] and B[k][i]; the result
+// D[i][j] is held by lane j + 16 (i % 4) in register i / 4.
+__device__ inline v4d mfma64(double a, double b, v4d c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
 struct DevPtr {
   void *p = nullptr;
   size_t bytes = 0;
@@ -130,6 +138,93 @@ __global__ void __launch_bounds__(256) k_triprod(const Group *__restrict__ group
       O[e] = (t == 0) ? acc : O[e] + acc;
     }
     __syncthreads();
+  }
+}
+
+constexpr int TP_MAXCT = 8;    // column tiles of 16 per strip (block sizes up to 128)
+
+// Block triple products on the matrix cores.  One wave per (group, 16-row
+// strip of the output block, frequency): T_strip = s op(A)_strip op(B) is
+// accumulated over column tiles with v_mfma_f64_16x16x4, parked in LDS, and
+// multiplied by op(C); all terms of the group accumulate in registers.
+__global__ void __launch_bounds__(64) k_triprod_mfma(const Group *__restrict__ groups, const Term *__restrict__ terms,
+                                                     const double *__restrict__ m0, const double *__restrict__ m1,
+                                                     const double *__restrict__ m2, const double *__restrict__ m3,
+                                                     const double *__restrict__ bbase, long bstride,
+                                                     double *__restrict__ obase, long ostride,
+                                                     const int *__restrict__ act, long nxy) {
+  __shared__ double sT[16][16 * TP_MAXCT + 1];
+  const Group g = groups[blockIdx.x];
+  const int strip = blockIdx.z;
+  const int w = act ? act[blockIdx.y] : (int)blockIdx.y;
+  const int i0 = strip * 16;
+  if (i0 >= g.nrow) return;                       // uniform per wave
+  const int lane = threadIdx.x, mi = lane & 15, mk = lane >> 4;
+  const double *B = bbase + (size_t)w * bstride;
+  double *O = obase + (size_t)w * ostride + (size_t)g.out_slot * nxy + g.out_off;
+  const double *mats[4] = {m0, m1, m2, m3};
+  const int nct = (g.ncol + 15) / 16;
+  v4d acc[TP_MAXCT];
+#pragma unroll
+  for (int c = 0; c < TP_MAXCT; ++c) acc[c] = v4d{0, 0, 0, 0};
+  const int irow = i0 + mi;                      // operand row of op(A) for this lane
+  const bool irow_ok = irow < g.nrow;
+  for (int t = 0; t < g.count; ++t) {
+    const Term tm = terms[g.first + t];
+    const double *A = mats[tm.a_id] + tm.a_off;
+    const double *Bb = B + (size_t)tm.b_slot * nxy + tm.b_off;
+    const double *C = mats[tm.c_id] + tm.c_off;
+    const int nda = tm.nda, ndc = tm.ndc;
+    // phase 1: T_strip = s op(A)[i0:i0+16, :] op(B)
+    v4d T[TP_MAXCT];
+#pragma unroll
+    for (int c = 0; c < TP_MAXCT; ++c) T[c] = v4d{0, 0, 0, 0};
+    for (int k0 = 0; k0 < nda; k0 += 4) {
+      const int k = k0 + mk;
+      const bool kok = k < nda;
+      const double av = (irow_ok && kok) ? (tm.ta ? A[k + (size_t)irow * nda] : A[irow + (size_t)k * nda]) : 0.0;
+#pragma unroll
+      for (int c = 0; c < TP_MAXCT; ++c) {
+        if (c >= nct) break;
+        const int j = c * 16 + mi;
+        const bool ok = kok && j < ndc;
+        const double bv = ok ? (tm.tb ? Bb[j + (size_t)k * tm.ldb] : Bb[k + (size_t)j * tm.ldb]) : 0.0;
+        T[c] = mfma64(av, bv, T[c]);
+      }
+    }
+    // park s*T in LDS: D[i][j] lives in lane j + 16 (i % 4), register i / 4
+#pragma unroll
+    for (int c = 0; c < TP_MAXCT; ++c) {
+      if (c >= nct) break;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) sT[4 * v + mk][c * 16 + mi] = tm.s * T[c][v];
+    }
+    __syncthreads();
+    // phase 2: acc += T_strip op(C)
+    for (int l0 = 0; l0 < ndc; l0 += 4) {
+      const int l = l0 + mk;
+      const bool lok = l < ndc;
+      const double tv = lok ? sT[mi][l] : 0.0;
+#pragma unroll
+      for (int c = 0; c < TP_MAXCT; ++c) {
+        if (c >= nct) break;
+        const int j = c * 16 + mi;
+        const bool ok = lok && j < ndc;
+        const double cv = ok ? (tm.tc ? C[j + (size_t)l * ndc] : C[l + (size_t)j * ndc]) : 0.0;
+        acc[c] = mfma64(tv, cv, acc[c]);
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int c = 0; c < TP_MAXCT; ++c) {
+    if (c >= nct) break;
+    const int j = c * 16 + mi;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int i = i0 + 4 * v + mk;
+      if (i < g.nrow && j < g.ncol) O[i + (size_t)j * g.nrow] = acc[c][v];
+    }
   }
 }
 
@@ -291,13 +386,6 @@ __global__ void __launch_bounds__(64) k_density(DensArgs A) {
   for (int d = 0; d < 2 * NDENS; ++d) out[(size_t)d * G] = acc[d] * wc;
 }
 
-typedef double v4d __attribute__((ext_vector_type(4)));
-
-// v_mfma_f64_16x16x4: lane i + 16k supplies A[i][k] and B[k][i]; the result
-// D[i][j] is held by lane j + 16 (i % 4) in register i / 4.
-__device__ inline v4d mfma64(double a, double b, v4d c) {
-  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-}
 
 // Accumulates the local-density contributions of one column b at one grid
 // point from T_x^s (x = psi, d_r psi, d_phi psi, d_z psi; s = spin of a) and
@@ -1134,6 +1222,7 @@ struct pnfam_gpu_ctx {
   DevPtr d_terms_q2s[2], d_groups_q2s[2], d_terms_s2q[2], d_groups_s2q[2];
   int ngroups_q2s[2] = {0, 0}, ngroups_s2q[2] = {0, 0};
   size_t lds_q2s = 0, lds_s2q = 0;
+  int strips = 1;   // 16-row strips of the largest output block
   DevPtr d_pairs;   // rho0, kap0, rho1, kap1 pair lists
   int pair_off[4] = {0, 0, 0, 0}, pair_n[4] = {0, 0, 0, 0};
   DevPtr d_tiles, d_tiles2;
@@ -1262,6 +1351,13 @@ int stage_qp2sp(pnfam_gpu_ctx *h, const double *vin, int na, const int *act) {
   Prof pr(h, K_TRIPROD_Q2S, na);
   for (int ri = 0; ri < 2; ++ri) {
     if (!h->ngroups_q2s[ri]) continue;
+    if (!h->basic_kernels) {
+      hipLaunchKernelGGL(k_triprod_mfma, dim3(h->ngroups_q2s[ri], na, h->strips), dim3(64), 0, h->stream,
+                         h->d_groups_q2s[ri].as<Group>(), h->d_terms_q2s[ri].as<Term>(), h->d_U[0].as<double>(),
+                         h->d_U[1].as<double>(), h->d_U[2].as<double>(), h->d_U[3].as<double>(), vin, 4 * nxy,
+                         h->d_rsp.as<double>(), 8 * nxy, act, nxy);
+      continue;
+    }
     hipLaunchKernelGGL(k_triprod, dim3(h->ngroups_q2s[ri], na), dim3(256), h->lds_q2s, h->stream,
                        h->d_groups_q2s[ri].as<Group>(), h->d_terms_q2s[ri].as<Term>(), h->d_U[0].as<double>(),
                        h->d_U[1].as<double>(), h->d_U[2].as<double>(), h->d_U[3].as<double>(), vin, 4 * nxy, 4,
@@ -1326,6 +1422,13 @@ int stage_sp2qp(pnfam_gpu_ctx *h, int na, const int *act) {
   Prof pr(h, K_TRIPROD_S2Q, na);
   for (int ri = 0; ri < 2; ++ri) {
     if (!h->ngroups_s2q[ri]) continue;
+    if (!h->basic_kernels) {
+      hipLaunchKernelGGL(k_triprod_mfma, dim3(h->ngroups_s2q[ri], na, h->strips), dim3(64), 0, h->stream,
+                         h->d_groups_s2q[ri].as<Group>(), h->d_terms_s2q[ri].as<Term>(), h->d_U[0].as<double>(),
+                         h->d_U[1].as<double>(), h->d_U[2].as<double>(), h->d_U[3].as<double>(),
+                         h->d_hsp.as<double>(), 8 * nxy, h->d_hqp.as<double>(), 4 * nxy, act, nxy);
+      continue;
+    }
     hipLaunchKernelGGL(k_triprod, dim3(h->ngroups_s2q[ri], na), dim3(256), h->lds_s2q, h->stream,
                        h->d_groups_s2q[ri].as<Group>(), h->d_terms_s2q[ri].as<Term>(), h->d_U[0].as<double>(),
                        h->d_U[1].as<double>(), h->d_U[2].as<double>(), h->d_U[3].as<double>(), h->d_hsp.as<double>(),
@@ -1584,6 +1687,12 @@ int pnfam_gpu_set_operator(pnfam_gpu_ctx *h, int32_t beta_minus, const int32_t *
     return e;
   // Fqp on the device with the same product machinery (one "frequency": B = f)
   h->nxy = nxy;
+  {
+    int dmax = 0;
+    for (int i = 0; i < nb; ++i) dmax = std::max(dmax, db[i]);
+    if (dmax > 16 * TP_MAXCT) return fail(PNFAM_GPU_EINVAL, "block size above 128 is not supported");
+    h->strips = (dmax + 15) / 16;
+  }
   {
     Planner p; p.db = db;
     SymBBM F1 = Fsp;

@@ -209,4 +209,13 @@ def test_calc_hamiltonian_plain_kernels(monkeypatch, K):
     got = eng.calc_hamiltonian(rsp[ARG])
     for k in range(8):
         assert rel(got[k], ref[k]) < 1e-12, k
+    rng = np.random.default_rng(60 + K)
+    x = [rng.standard_normal(prob.nxy) for _ in range(4)]
+    ref_rsp, _ = orc.qp_to_sp(*x)
+    assert rel(eng.qp2sp(*x), ref_rsp) < 1e-13
+    hsp = rng.standard_normal((8, prob.nxy))
+    ref_q = orc.sp_to_qp(hsp[ARG], np.array([fm, xm, ym, fm]))
+    got_q = eng.sp2qp(hsp)
+    for k in range(4):
+        assert rel(got_q[k], ref_q[k]) < 1e-13
     eng.close()
