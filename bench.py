@@ -61,39 +61,51 @@ def cpu_baseline(prob, mean_iters, iters_sample, threads):
                       f"= {s_per_iter:.2f} s/iteration, scaled by the GPU mean of {mean_iters:.1f} iterations/point"}
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--nsh", type=int, default=16)
-    ap.add_argument("--beta", type=float, default=0.3)
-    ap.add_argument("--ngrid", type=int, default=40)
-    ap.add_argument("--npts", type=int, default=128, help="frequencies per K per step")
-    ap.add_argument("--max-iter", type=int, default=300)
-    ap.add_argument("--cpu-iters", type=int, default=2, help="oracle iterations in the CPU sample (0: skip)")
-    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(json.dumps({"error": "this engine runs one frequency batch per GPU; only --gpus 1 is measured "
-                                   "(points are independent: N GPUs would each take npts/N points)"}))
-        sys.exit(1)
+def copy_calibration(mib):
+    """One plain device copy of known size: its FETCH_SIZE / WRITE_SIZE readings
+    convert the counters of the engine kernels to HBM bytes."""
+    import torch
+    n = mib << 20
+    x = torch.ones(n, dtype=torch.uint8, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": n}), flush=True)
 
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    (profiles/<kernel>_traffic_*.txt, line 'bytes_per_launch: <value>'), or None."""
+    d = os.path.join(ROOT, "profiles")
+    if not os.path.isdir(d):
+        return None
+    for f in sorted(os.listdir(d)):
+        if f.startswith(kernel + "_traffic") and f.endswith(".txt"):
+            with open(os.path.join(d, f)) as fh:
+                for line in fh:
+                    if line.startswith("bytes_per_launch:"):
+                        return float(line.split(":")[1])
+    return None
+
+
+# peak of the arithmetic each kernel issues (MI355X dense figures)
+KERNEL_PEAK = {"mix_gemm_density": (2500.0, "TFLOP/s"), "mix_gemm_project": (2500.0, "TFLOP/s")}
+
+
+def measure(engines, w, precision, args):
+    """Warm-up and timed steps of one precision path; per-kernel HIP-event totals."""
     import numpy as np
-    from pynfam_amd.engine import GpuEngine
-
-    basis, probs = build_workload(args.nsh, args.beta, 60, 92, args.ngrid)
-    engines = [GpuEngine(p) for p in probs]
-    w = contour(args.npts)
 
     def step():
-        conv, its = 0, []
+        conv, its, S = 0, [], []
         for e in engines:
-            r = e.solve(w, max_iter=args.max_iter)
+            r = e.solve(w, max_iter=args.max_iter, precision=precision)
             ok = r["iterations"] >= 0
             conv += int(ok.sum())
             its.extend(r["iterations"][ok].tolist())
-        return conv, its
+            S.append(r["strength"])
+        return conv, its, np.concatenate(S)
 
     for _ in range(args.warmup):
         step()
@@ -102,7 +114,7 @@ def main():
     t0 = time.perf_counter()
     conv_total, its_all = 0, []
     for _ in range(args.steps):
-        c, its = step()
+        c, its, S = step()
         conv_total += c
         its_all.extend(its)
     dt = time.perf_counter() - t0
@@ -115,30 +127,92 @@ def main():
             d["units"] += v["units"]
             d["flops"] += v["flops_per_unit"] * v["units"]
         e.profile(False)
-    total_pts = args.steps * len(engines) * len(w)
-    value = conv_total / dt
-    mean_iters = float(np.mean(its_all)) if its_all else float("nan")
-    dom = max(prof.items(), key=lambda kv: kv[1]["ms"])
-    name, d = dom
+    flop_kernels = {k: v for k, v in prof.items() if v["flops"] > 0}
+    return {"value": conv_total / dt, "ms_per_step": dt * 1e3 / args.steps, "converged": conv_total,
+            "total": args.steps * len(engines) * len(w),
+            "mean_iterations": float(np.mean(its_all)) if its_all else float("nan"),
+            "dominant": max(flop_kernels.items(), key=lambda kv: kv[1]["ms"]),
+            "kernel_ms": {k: round(v["ms"], 3) for k, v in sorted(prof.items(), key=lambda kv: -kv[1]["ms"])
+                          if v["ms"] > 0},
+            "strength": S}
+
+
+def rate_error(w, s_ref, s_mix, nk):
+    """Relative error of the contour-integrated rate when the mixed strengths
+    replace the FP64 ones (None until a phase space is supplied)."""
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--nsh", type=int, default=16)
+    ap.add_argument("--beta", type=float, default=0.3)
+    ap.add_argument("--ngrid", type=int, default=40)
+    ap.add_argument("--npts", type=int, default=128, help="frequencies per K per step")
+    ap.add_argument("--max-iter", type=int, default=300)
+    ap.add_argument("--cpu-iters", type=int, default=10, help="oracle iterations in the CPU sample (0: skip)")
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--precision", choices=["fp64", "mixed", "mixed_refine", "both"], default="both",
+                    help="both: FP64 then mixed; the JSON value is the last one")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="only run a device-to-device copy of this many MiB (calibrates FETCH_SIZE/WRITE_SIZE)")
+    args = ap.parse_args()
+    if args.copy_calibration_mib > 0:
+        copy_calibration(args.copy_calibration_mib)
+        return
+    if args.gpus != 1:
+        print(json.dumps({"error": "this engine runs one frequency batch per GPU; only --gpus 1 is measured "
+                                   "(points are independent: N GPUs would each take npts/N points)"}))
+        sys.exit(1)
+
+    import numpy as np
+    from pynfam_amd import engine as E
+
+    basis, probs = build_workload(args.nsh, args.beta, 60, 92, args.ngrid)
+    engines = [E.GpuEngine(p) for p in probs]
+    w = contour(args.npts)
+    modes = {"fp64": [E.FP64], "mixed": [E.MIXED_NOREFINE], "mixed_refine": [E.MIXED],
+             "both": [E.FP64, E.MIXED_NOREFINE]}[args.precision]
+    runs = {}
+    for prec in modes:
+        runs[prec] = measure(engines, w, prec, args)
+    head = runs[modes[-1]]
+    name, d = head["dominant"]
+    peak, unit = KERNEL_PEAK.get(name, (PEAK_FP64_TFLOPS, "TFLOP/s"))
     achieved = d["flops"] / (d["ms"] * 1e-3) / 1e12 if d["ms"] > 0 else 0.0
-    roofline = {"kernel": name, "bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
-                "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+    roofline = {"kernel": name, "bound": "mfma", "achieved": achieved, "peak": peak,
+                "unit": unit, "frac": achieved / peak, "traffic": measured_traffic(name),
                 "avg_launch_ms": d["ms"] / max(d["launches"], 1), "launches": d["launches"],
                 "algorithmic_flop_per_launch": d["flops"] / max(d["launches"], 1)}
-    breakdown = {k: round(v["ms"], 3) for k, v in sorted(prof.items(), key=lambda kv: -kv[1]["ms"]) if v["ms"] > 0}
     cpu = None
     if args.cpu_iters > 0:
-        cpu = cpu_baseline(probs[0], mean_iters, args.cpu_iters, args.cpu_threads)
-    out = {"metric": METRIC, "value": value, "unit": "points/s", "n_gpus": 1, "steps": args.steps,
-           "warmup": args.warmup, "ms_per_step": dt * 1e3 / args.steps, "higher_is_better": True,
-           "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        cpu = cpu_baseline(probs[0], runs[modes[0]]["mean_iterations"], args.cpu_iters, args.cpu_threads)
+    mixed_name = {E.FP64: "FP64", E.MIXED: "mixed (split-BF16 grid stage, FP64 refinement)",
+                  E.MIXED_NOREFINE: "mixed (split-BF16 grid stage on FP64-accumulated increments)"}
+    out = {"metric": METRIC, "value": head["value"], "unit": "points/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": head["ms_per_step"], "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None,
+           "dtype": "f64" if modes[-1] == E.FP64 else "bf16x3/fp32-acc grid stage, f64 elsewhere",
+           "data": "synthetic",
            "config": {"workload": f"pnFAM GT- K=0,1, synthetic HFB Z=60 N=92 beta2={args.beta} N_sh={args.nsh} "
                                   f"{args.ngrid}x{args.ngrid} grid, {len(w)} contour points per K per step, "
-                                  f"FP64 path, tol 1e-7, Broyden M=50",
+                                  f"{mixed_name[modes[-1]]}, tol 1e-7, Broyden M=50",
                       "dqp": int(basis.dqp), "nxy_K0": engines[0].nxy, "nxy_K1": engines[1].nxy,
                       "points_per_step": len(engines) * len(w)},
-           "points_converged": conv_total, "points_total": total_pts, "mean_iterations": mean_iters,
-           "roofline": roofline, "kernel_ms": breakdown, "cpu_baseline": cpu}
+           "points_converged": head["converged"], "points_total": head["total"],
+           "mean_iterations": head["mean_iterations"],
+           "roofline": roofline, "kernel_ms": head["kernel_ms"], "cpu_baseline": cpu}
+    if len(modes) == 2:
+        ref, mix = runs[modes[0]], runs[modes[1]]
+        err = np.abs(mix["strength"] - ref["strength"]) / np.abs(ref["strength"])
+        out["fp64"] = {"value": ref["value"], "ms_per_step": ref["ms_per_step"],
+                       "mean_iterations": ref["mean_iterations"], "kernel_ms": ref["kernel_ms"]}
+        out["mixed_vs_fp64"] = mix["value"] / ref["value"]
+        out["strength_rel_err_vs_fp64"] = {"max": float(err.max()), "median": float(np.median(err))}
+        out["rate_rel_err_vs_fp64"] = rate_error(w, ref["strength"], mix["strength"], len(engines))
     print(json.dumps(out), flush=True)
 
 

@@ -57,11 +57,23 @@ typedef struct {
   int32_t broyden_history;    /* M; <0 disables mixing */
   double quench;              /* quench_residual_int; 0 skips the residual interaction */
   double alpha_mix;           /* linear mixing factor of the Broyden step */
-  int32_t precision;          /* PNFAM_GPU_FP64 or PNFAM_GPU_MIXED */
+  int32_t precision;          /* PNFAM_GPU_FP64, PNFAM_GPU_MIXED or PNFAM_GPU_MIXED_NOREFINE */
 } pnfam_gpu_opts;
 
+/* Precision of the residual interaction (the reference computes everything in
+ * FP64; pnfam_solver.f90:93-221).
+ *   FP64: every step in FP64.
+ *   MIXED: the hamiltonian is linear in the amplitudes, so H(v_k) is carried as
+ *     an FP64 sum of H applied to the increments v_k - v_{k-1}; each increment goes
+ *     through the grid stage on split-BF16 MFMA with FP32 accumulation.  The
+ *     residual, its max-norm, Broyden, G(w) and S stay FP64.  When a point first
+ *     meets the tolerance, one full FP64 evaluation replaces the accumulated H
+ *     and the point iterates on until it meets the tolerance again.
+ *   MIXED_NOREFINE: as MIXED without the final FP64 evaluation; S then carries
+ *     the low-precision error of the first increments (about 1e-5 relative). */
 #define PNFAM_GPU_FP64 0
 #define PNFAM_GPU_MIXED 1
+#define PNFAM_GPU_MIXED_NOREFINE 2
 
 /* Copies the basis to the device and builds the resident tables.
  * Replaces setup_pnfam's HFB + interaction set-up (pnfam_setup.f90:134-174). */
@@ -108,6 +120,13 @@ int pnfam_gpu_calc_hamiltonian(pnfam_gpu_ctx *h, const double *in, double *out);
  * hsp as the out array of pnfam_gpu_calc_hamiltonian. */
 int pnfam_gpu_sp2qp(pnfam_gpu_ctx *h, const double *hsp, double *h20_re, double *h20_im, double *h02_re,
                     double *h02_im);
+
+/* Selects the grid stage used by pnfam_gpu_calc_hamiltonian and
+ * pnfam_gpu_densities: PNFAM_GPU_FP64 (default) or PNFAM_GPU_MIXED (the
+ * split-BF16 pair-table contractions that the mixed solve applies to its
+ * increments).  No counterpart in the reference: a parity hook for the mixed
+ * grid stage. */
+int pnfam_gpu_set_unit_precision(pnfam_gpu_ctx *h, int32_t precision);
 
 /* Coordinate-space densities of one sector (diagnostic): dens [56][nghl],
  * 28 complex local densities (re, im interleaved by density) after the

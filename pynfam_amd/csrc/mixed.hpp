@@ -1,0 +1,386 @@
+// Split-BF16 grid stage of the mixed-precision path (device side).
+//
+// The grid stage of calc_hamiltonian (pnfam_hamiltonian_blas.f90:124-1262) is
+// two contractions between the block elements k = (a, b) of a block matrix and
+// the grid points r:
+//   densities   D_c^cls(r, n) = sum_{k in cls} psi_x(r,a) psi_y(r,b) rho_k(n)
+//   projection  h_k(n)        = sum_{c, r} psi_z(r,a) psi_z'(r,b) F_c^cls(k)(r, n)
+// with c = (x, y) or (z, z') a pair of wavefunction tables, cls = (s_a, s_b) the
+// spin class of the element and n a column (frequency, sector, re/im).  The
+// products psi psi are omega-independent, so they are built once per operator
+// as resident tables ("pair tables", a few GB in HBM) and both contractions
+// become plain GEMMs whose large operand is streamed from HBM once per pass
+// and shared by every frequency of the batch.  Operands are split into BF16
+// hi + lo parts; three MFMA products (hi*hi + hi*lo + lo*hi) accumulate in FP32,
+// which keeps about 16 significant bits of every product.
+//
+// Element order k: class-major (cls = 2 s_a + s_b), blocks in row order inside a
+// class, column-major inside a block; each class range is padded to a multiple
+// of MIX_KALIGN with zero table entries.
+
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+constexpr int MIX_KALIGN = 128;   // class ranges padded to this many elements
+constexpr int MIX_RALIGN = 32;    // grid points padded to this
+constexpr int GT_ROWS = 128;      // table rows per workgroup
+constexpr int GS_ROWS = 256;      // small-operand rows (columns n) per workgroup
+constexpr int GKS = 32;           // K per LDS stage (one MFMA K step)
+constexpr int GLDP = 40;          // LDS row pitch in bf16 (80 bytes)
+
+// The 18 table pairs (z, z') with non-zero mean fields (the assignments of
+// pnfam_hamiltonian_blas.f90:774-1093 never fill z or z' = 4 together with 1..4).
+__constant__ int8_t kProjZ[18][2] = {{0, 0}, {0, 1}, {1, 0}, {0, 2}, {2, 0}, {0, 3}, {3, 0}, {0, 4}, {4, 0},
+                                     {1, 1}, {1, 2}, {2, 1}, {1, 3}, {3, 1}, {2, 2}, {2, 3}, {3, 2}, {3, 3}};
+constexpr int NPROJ_C = 18;
+constexpr int NDENS_C = 16;   // (x, y) in {0..3}^2, c = 4 x + y
+
+__device__ inline void split_bf16(float v, __bf16 &hi, __bf16 &lo) {
+  hi = (__bf16)v;
+  lo = (__bf16)(v - (float)hi);
+}
+
+// ---- pair tables --------------------------------------------------------
+struct TabArgs {
+  const double *psi[5];       // [state][r]
+  const int *sa, *sb;         // [Kp] states of the element, -1 for padding
+  int Kp, nghl, RP;
+  int nc;                     // combos
+  int dens;                   // 1: density layout [c][r][k], 0: projection layout [c][k][r]
+  int single;                 // 1: the (0,0) combo only
+  __bf16 *hi, *lo;
+};
+
+// One thread per (k, r) pair of one combo (blockIdx.y).  Projection layout is
+// written with r fastest, density layout with k fastest.
+__global__ void k_mix_tables(TabArgs T) {
+  const int c = blockIdx.y;
+  int x, y;
+  if (T.single) { x = 0; y = 0; }
+  else if (T.dens) { x = c >> 2; y = c & 3; }
+  else { x = kProjZ[c][0]; y = kProjZ[c][1]; }
+  const size_t n = (size_t)T.Kp * T.RP;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+    int k, r;
+    if (T.dens) { k = (int)(e % T.Kp); r = (int)(e / T.Kp); }
+    else { r = (int)(e % T.RP); k = (int)(e / T.RP); }
+    const int a = T.sa[k], b = T.sb[k];
+    float v = 0.0f;
+    if (a >= 0 && r < T.nghl)
+      v = (float)(T.psi[x][(size_t)a * T.nghl + r] * T.psi[y][(size_t)b * T.nghl + r]);
+    __bf16 h, l;
+    split_bf16(v, h, l);
+    const size_t o = (size_t)c * n + e;
+    T.hi[o] = h;
+    T.lo[o] = l;
+  }
+}
+
+// ---- operand conversion -------------------------------------------------
+// Bd[n][k] (k fastest) from the FP64 block matrices in the reference layout.
+// n = (slot * nsec + sec) * 2 + ri; sector sec reads quadrant q_of_sec[sec].
+struct CvtRhoArgs {
+  const double *rsp;          // [w][8][nxy]
+  const int *elem;            // [Kp]
+  const int *act;
+  long nxy;
+  int Kp, nsec, q0, q1, ncap;
+  __bf16 *hi, *lo;            // [ncap][Kp]
+};
+
+__global__ void k_mix_rho(CvtRhoArgs A) {
+  const int slot = blockIdx.y;
+  const int w = A.act[slot];
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < A.Kp; k += gridDim.x * blockDim.x) {
+    const int e = A.elem[k];
+    for (int sec = 0; sec < A.nsec; ++sec) {
+      const int q = sec ? A.q1 : A.q0;
+      for (int ri = 0; ri < 2; ++ri) {
+        const float v = e >= 0 ? (float)A.rsp[((size_t)w * 8 + 2 * q + ri) * A.nxy + e] : 0.0f;
+        __bf16 h, l;
+        split_bf16(v, h, l);
+        const size_t o = (size_t)((slot * A.nsec + sec) * 2 + ri) * A.Kp + k;
+        A.hi[o] = h;
+        A.lo[o] = l;
+      }
+    }
+  }
+}
+
+// F[c][cls][n][r] (r fastest) from the FP64 fields [w][sec][208][r], times the
+// factor 2 of the projection (pnfam_hamiltonian_blas.f90:1106-1160, 1240-1262).
+// pair != 0: the four pairing fields of one sector, F[cls][n][r] with
+// n = slot * 2 + ri.  Otherwise the 18 mean-field pairs for both sectors.
+struct CvtFldArgs {
+  const double *flds;
+  const int *act;
+  int nghl, RP, ncap;
+  int pair, sec;
+  __bf16 *hi, *lo;
+};
+
+__global__ void k_mix_fields(CvtFldArgs A) {
+  const int slot = blockIdx.y;
+  const int w = A.act[slot];
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= A.RP) return;
+  const bool live = r < A.nghl;
+  const size_t plane = (size_t)A.ncap * A.RP;
+  if (A.pair) {
+    const double *f = A.flds + (((size_t)w * 2 + A.sec) * 2 * NFLD) * A.nghl;
+    for (int cls = 0; cls < 4; ++cls) {
+      const int sa = cls >> 1, sb = cls & 1;
+      // pairing field index of (s_a, s_b): (up,up) 0, (down,up) 1, (up,down) 2, (down,down) 3
+      const int idx = 100 + (sb == 0 ? (sa == 0 ? 0 : 1) : (sa == 0 ? 2 : 3));
+      for (int ri = 0; ri < 2; ++ri) {
+        const float v = live ? (float)(2.0 * f[(size_t)(2 * idx + ri) * A.nghl + r]) : 0.0f;
+        __bf16 h, l;
+        split_bf16(v, h, l);
+        const size_t o = (size_t)cls * plane + (size_t)(slot * 2 + ri) * A.RP + r;
+        A.hi[o] = h;
+        A.lo[o] = l;
+      }
+    }
+    return;
+  }
+  for (int sec = 0; sec < 2; ++sec) {
+    const double *f = A.flds + (((size_t)w * 2 + sec) * 2 * NFLD) * A.nghl;
+    for (int c = 0; c < NPROJ_C; ++c) {
+      const int z = kProjZ[c][0], zp = kProjZ[c][1];
+      for (int cls = 0; cls < 4; ++cls) {
+        const int idx = mf_index(z, zp, cls >> 1, cls & 1);
+        for (int ri = 0; ri < 2; ++ri) {
+          const float v = live ? (float)(2.0 * f[(size_t)(2 * idx + ri) * A.nghl + r]) : 0.0f;
+          __bf16 h, l;
+          split_bf16(v, h, l);
+          const size_t o = (size_t)(c * 4 + cls) * plane + (size_t)((slot * 2 + sec) * 2 + ri) * A.RP + r;
+          A.hi[o] = h;
+          A.lo[o] = l;
+        }
+      }
+    }
+  }
+}
+
+// ---- the split GEMM -------------------------------------------------------
+// O[t][s] = sum_{seg, kk} T[t][seg, kk] * S[s][seg, kk] with both operands
+// K-contiguous BF16 hi/lo.  One job per spin class.
+struct GemmJob {
+  int t0, tcount;     // table rows of this job
+  int k0, klen;       // K range inside each segment (klen multiple of GKS)
+  int cls, pad;       // spin class: selects the small operand and output planes
+};
+
+struct GemmArgs {
+  const __bf16 *Th, *Tl;
+  long ldT, segT;
+  const __bf16 *Sh, *Sl;
+  long ldS, segS;
+  int nseg, nS, nst;
+  long s_cls, o_cls;  // per-class offsets of the small operand and of the output
+  const GemmJob *jobs;
+  int mode;           // 0: FP32 O[t][s]; 1: FP64 scatter into the block matrices
+  float *O;
+  long ldO;
+  double *hsp;        // mode 1: [w][8][nxy]
+  const int *elem;    // mode 1: [Kp]
+  const int *act;
+  long nxy;
+  int nsec, q0, q1;
+};
+
+// Workgroup: 8 waves, 128 table rows x 256 columns; each wave 64 x 64 (4 x 4
+// MFMA tiles of v_mfma_f32_16x16x32_bf16).  The flat block index is dealt to
+// the 8 XCDs round-robin; it is decoded so that all column tiles of one table
+// tile run on one XCD and share that table tile through its L2.
+__global__ void __launch_bounds__(512) k_mix_gemm(GemmArgs G) {
+  __shared__ __attribute__((aligned(16))) __bf16 sT[2][GT_ROWS][GLDP];
+  __shared__ __attribute__((aligned(16))) __bf16 sS[2][GS_ROWS][GLDP];
+  const GemmJob J = G.jobs[blockIdx.y];
+  const int b = blockIdx.x;
+  const int xcd = b & 7, loc = b >> 3;
+  const int st = loc % G.nst;
+  const int tt = (loc / G.nst) * 8 + xcd;
+  if (tt * GT_ROWS >= J.tcount) return;   // whole workgroup leaves before any barrier
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int wt = wave & 1, ws = wave >> 1;
+  const int fi = lane & 15, fg = lane >> 4;
+  // wave-uniform activity of the 4 column tiles of this wave
+  const int s_wave0 = st * GS_ROWS + ws * 64;
+  int nj = (G.nS - s_wave0 + 15) / 16;
+  nj = nj < 0 ? 0 : (nj > 4 ? 4 : nj);
+  int ni = (J.tcount - (tt * GT_ROWS + wt * 64) + 15) / 16;
+  ni = ni < 0 ? 0 : (ni > 4 ? 4 : ni);
+
+  v4f acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
+
+  // staging assignments: table tile 128 x 32 = 512 chunks of 8 (one per thread),
+  // small tile 256 x 32 = 1024 chunks (two per thread)
+  const int trow = tid >> 2, tch = tid & 3;
+  const bool t_ok = tt * GT_ROWS + trow < J.tcount;
+  const long t_base = (long)(J.t0 + tt * GT_ROWS + (t_ok ? trow : 0)) * G.ldT + J.k0 + tch * 8;
+  const long s_off = (long)J.cls * G.s_cls;
+  const long o_off = (long)J.cls * G.o_cls;
+  int srow[2], sch[2];
+  bool s_ok[2];
+  long s_base[2];
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    const int cidx = tid + q * 512;
+    srow[q] = cidx >> 2; sch[q] = cidx & 3;
+    const int s = st * GS_ROWS + srow[q];
+    s_ok[q] = s < G.nS;
+    s_base[q] = s_off + (long)(s_ok[q] ? s : 0) * G.ldS + J.k0 + sch[q] * 8;
+  }
+  const v8bf z8 = {0, 0, 0, 0, 0, 0, 0, 0};
+  v8bf rth, rtl, rsh[2], rsl[2];
+  auto load = [&](int sg, int kk) {
+    const long to = t_base + (long)sg * G.segT + kk;
+    rth = t_ok ? *(const v8bf *)(G.Th + to) : z8;
+    rtl = t_ok ? *(const v8bf *)(G.Tl + to) : z8;
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const long so = s_base[q] + (long)sg * G.segS + kk;
+      rsh[q] = s_ok[q] ? *(const v8bf *)(G.Sh + so) : z8;
+      rsl[q] = s_ok[q] ? *(const v8bf *)(G.Sl + so) : z8;
+    }
+  };
+  auto store = [&]() {
+    *(v8bf *)&sT[0][trow][tch * 8] = rth;
+    *(v8bf *)&sT[1][trow][tch * 8] = rtl;
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      *(v8bf *)&sS[0][srow[q]][sch[q] * 8] = rsh[q];
+      *(v8bf *)&sS[1][srow[q]][sch[q] * 8] = rsl[q];
+    }
+  };
+  const int nk = J.klen / GKS;
+  const int nsteps = G.nseg * nk;
+  if (nsteps > 0) {
+    load(0, 0);
+    store();
+  }
+  __syncthreads();
+  for (int step = 0; step < nsteps; ++step) {
+    const bool more = step + 1 < nsteps;
+    if (more) load((step + 1) / nk, ((step + 1) % nk) * GKS);   // in flight during the MFMAs
+    if (ni > 0 && nj > 0) {
+      v8bf ah[4], al[4], bh[4], bl[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        ah[i] = *(const v8bf *)&sT[0][wt * 64 + i * 16 + fi][fg * 8];
+        al[i] = *(const v8bf *)&sT[1][wt * 64 + i * 16 + fi][fg * 8];
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        bh[j] = *(const v8bf *)&sS[0][ws * 64 + j * 16 + fi][fg * 8];
+        bl[j] = *(const v8bf *)&sS[1][ws * 64 + j * 16 + fi][fg * 8];
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        if (i >= ni) break;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          if (j >= nj) break;
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
+        }
+      }
+    }
+    __syncthreads();
+    if (more) {
+      store();
+      __syncthreads();
+    }
+  }
+  // epilogue: lane holds D[t = 4 (lane / 16) + v][s = lane % 16] of every tile
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    if (i >= ni) break;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      if (j >= nj) break;
+      const int s = s_wave0 + j * 16 + (lane & 15);
+      if (s >= G.nS) continue;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int tl = tt * GT_ROWS + wt * 64 + i * 16 + 4 * (lane >> 4) + v;
+        if (tl >= J.tcount) continue;
+        if (G.mode == 0) {
+          G.O[o_off + (long)tl * G.ldO + s] = acc[i][j][v];
+        } else {
+          const int e = G.elem[J.t0 + tl];
+          if (e < 0) continue;
+          const int ri = s & 1;
+          const int sec = G.nsec == 2 ? (s >> 1) & 1 : 0;
+          const int slot = s / (2 * G.nsec);
+          const int q = sec ? G.q1 : G.q0;
+          G.hsp[((size_t)G.act[slot] * 8 + 2 * q + ri) * G.nxy + e] = (double)acc[i][j][v];
+        }
+      }
+    }
+  }
+}
+
+// ---- densities from the class-summed contractions -------------------------
+// Of[cls][c][r][n] (mean-field pairs c = 4x + y, n = (slot*2 + sec)*2 + ri) and
+// Ok{0,1}[cls][r][n'] (pairing, n' = slot*2 + ri) -> dens[w][sec][56][r],
+// with the products of pnfam_hamiltonian_blas.f90:255-581 per spin of b.
+struct CombArgs {
+  const float *Of, *Ok0, *Ok1;
+  long ldf, ldk;             // row pitch (columns) of Of and Ok
+  int RP, nghl, na;
+  const double *wdcori;
+  double *dens;              // [w][2][56][nghl]
+  const int *act;
+};
+
+// One thread per (grid point, frequency slot, sector); threads of a block run
+// along (slot, sector), which is the contiguous direction of Of and Ok.
+__global__ void __launch_bounds__(128) k_mix_dens(CombArgs A) {
+  const int r = blockIdx.x;
+  const int j = blockIdx.y * 128 + threadIdx.x;
+  if (j >= 2 * A.na) return;
+  const int slot = j >> 1, sec = j & 1;
+  const int w = A.act[slot];
+  double acc[2 * NDENS];
+#pragma unroll
+  for (int d = 0; d < 2 * NDENS; ++d) acc[d] = 0.0;
+  const size_t plane_f = (size_t)A.RP * A.ldf;
+  const size_t plane_k = (size_t)A.RP * A.ldk;
+  const float *Ok = sec ? A.Ok1 : A.Ok0;
+  for (int sb = 0; sb < 2; ++sb) {
+    const int s = sb == 0 ? 1 : -1;
+    const int cd = 2 * sb + sb, co = 2 * (1 - sb) + sb;   // (s_a = s_b, s_b), (s_a != s_b, s_b)
+    double Pr[4][4], Pi[4][4], Qr[4][4], Qi[4][4];
+#pragma unroll
+    for (int x = 0; x < 4; ++x)
+#pragma unroll
+      for (int y = 0; y < 4; ++y) {
+        const int c = 4 * x + y;
+        const float *pd = A.Of + (size_t)(cd * NDENS_C + c) * plane_f + (size_t)r * A.ldf + 2 * j;
+        const float *po = A.Of + (size_t)(co * NDENS_C + c) * plane_f + (size_t)r * A.ldf + 2 * j;
+        Pr[x][y] = pd[0]; Pi[x][y] = pd[1];
+        Qr[x][y] = po[0]; Qi[x][y] = po[1];
+      }
+    density_accumulate(acc, Pr, Pi, Qr, Qi, s);
+    // pairing densities (pnfam_hamiltonian_blas.f90:584-640)
+    const float *kd = Ok + (size_t)cd * plane_k + (size_t)r * A.ldk + 2 * slot;
+    const float *ko = Ok + (size_t)co * plane_k + (size_t)r * A.ldk + 2 * slot;
+    double pr = 2.0 * ko[0], pi = 2.0 * ko[1];
+    acc[2 * D_RB] -= s * pr; acc[2 * D_RB + 1] -= s * pi;
+    acc[2 * D_SBZ] += pr; acc[2 * D_SBZ + 1] += pi;
+    pr = 2.0 * kd[0]; pi = 2.0 * kd[1];
+    acc[2 * D_SBR] += s < 0 ? pr : -pr; acc[2 * D_SBR + 1] += s < 0 ? pi : -pi;
+    acc[2 * D_SBP] += pi; acc[2 * D_SBP + 1] -= pr;
+  }
+  const double wc = A.wdcori[r];
+  double *out = A.dens + (((size_t)w * 2 + sec) * 2 * NDENS) * A.nghl + r;
+#pragma unroll
+  for (int d = 0; d < 2 * NDENS; ++d) out[(size_t)d * A.nghl] = acc[d] * wc;
+}

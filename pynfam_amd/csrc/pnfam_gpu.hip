@@ -390,18 +390,12 @@ __global__ void __launch_bounds__(64) k_density(DensArgs A) {
 // Accumulates the local-density contributions of one column b at one grid
 // point from T_x^s (x = psi, d_r psi, d_phi psi, d_z psi; s = spin of a) and
 // psi_y(b) (pnfam_hamiltonian_blas.f90:255-581).
-__device__ inline void density_column(double *acc, const double (&Tr)[4][2], const double (&Ti)[4][2],
-                                      const double (&y)[4], int s) {
+// P[x][y] = sum over a of the spin of b, Q[x][y] = sum over a of the other
+// spin, of psi_x(a) rho(a, b) psi_y(b), for columns b of spin s (+1/-1).
+template <class R>
+__device__ inline void density_accumulate(double *acc, const R (&Pr)[4][4], const R (&Pi)[4][4],
+                                          const R (&Qr)[4][4], const R (&Qi)[4][4], int s) {
 #define ACC(d, ri) acc[2 * (d) + (ri)]
-  const int sd = s > 0 ? 0 : 1, so = 1 - sd;
-  double Pr[4][4], Pi[4][4], Qr[4][4], Qi[4][4];
-#pragma unroll
-  for (int x = 0; x < 4; ++x)
-#pragma unroll
-    for (int yy = 0; yy < 4; ++yy) {
-      Pr[x][yy] = Tr[x][sd] * y[yy]; Pi[x][yy] = Ti[x][sd] * y[yy];
-      Qr[x][yy] = Tr[x][so] * y[yy]; Qi[x][yy] = Ti[x][so] * y[yy];
-    }
   double ar, ai;
   ACC(D_RHO, 0) += Pr[0][0]; ACC(D_RHO, 1) += Pi[0][0];
   ACC(D_SZ, 0) += s * Pr[0][0]; ACC(D_SZ, 1) += s * Pi[0][0];
@@ -449,6 +443,20 @@ __device__ inline void density_column(double *acc, const double (&Tr)[4][2], con
   ai = Qi[3][1] + Qi[1][3] + e * (Qi[3][2] - Qi[2][3]);
   ACC(D_FZ, 0) += 0.5 * ar; ACC(D_FZ, 1) += 0.5 * ai;
 #undef ACC
+}
+
+__device__ inline void density_column(double *acc, const double (&Tr)[4][2], const double (&Ti)[4][2],
+                                      const double (&y)[4], int s) {
+  const int sd = s > 0 ? 0 : 1, so = 1 - sd;
+  double Pr[4][4], Pi[4][4], Qr[4][4], Qi[4][4];
+#pragma unroll
+  for (int x = 0; x < 4; ++x)
+#pragma unroll
+    for (int yy = 0; yy < 4; ++yy) {
+      Pr[x][yy] = Tr[x][sd] * y[yy]; Pi[x][yy] = Ti[x][sd] * y[yy];
+      Qr[x][yy] = Tr[x][so] * y[yy]; Qi[x][yy] = Ti[x][so] * y[yy];
+    }
+  density_accumulate(acc, Pr, Pi, Qr, Qi, s);
 }
 
 // Local densities with FP64 MFMA (v_mfma_f64_16x16x4).  One wave per
@@ -711,6 +719,8 @@ __global__ void __launch_bounds__(128) k_fields(FieldArgs F) {
   put(100 + 3, cspair * (-1.0 * sbr - mul_i(sbp)));      // (down, down)
 }
 
+#include "mixed.hpp"
+
 struct Tile {        // output tile of one block of a projected matrix
   int32_t ix, iy, off, sec;   // row block, col block, element offset, sector
   int32_t a0, b0, na, nb_;    // tile origin / size inside the block
@@ -837,6 +847,9 @@ constexpr int P2_MAXRT = 4;   // row tiles of 16 per wave
 // B_z(r,b) = sum_z' mf(z,z',s_a,s_b)(r) psi_z'(r,b) built in registers from
 // fields staged in LDS per 16 grid points (kind 1: B = P_{s_a s_b}(r) psi_0(r,b)).
 // Rows: up to 64 states of spin s_a of block ix; columns: 16 states of block iy.
+// Grid (tile, frequency), tiles fastest.  Ordering frequencies fastest within
+// an XCD (so a tile's psi rows are fetched once per L2) cuts the HBM fetch 4x
+// but ran 1.4x slower: the kernel is not bound by HBM traffic.
 __global__ void __launch_bounds__(64) k_project_mfma(ProjArgs P, const Tile2 *__restrict__ tiles) {
   __shared__ double sMF[16][2][5][5][2];   // [r][s_b][z][z'][ri]
   const Tile2 t = tiles[blockIdx.x];
@@ -969,6 +982,39 @@ __global__ void k_green(GreenArgs A) {
     v[(isx ? 0 : n) + k] = gr * hr - gi * hi;
     v[(isx ? 2 * n : 3 * n) + k] = gi * hr + hi * gr;
   }
+}
+
+// ---- mixed path: the hamiltonian is linear in vin, so H(vin_k) is carried as
+// an FP64 sum of H applied to the increments vin_k - vin_{k-1} --------------
+// dvin = vin - vprev, vprev = vin (per active frequency, n doubles each)
+__global__ void k_delta(const double *__restrict__ vin, double *__restrict__ vprev, double *__restrict__ dvin,
+                        long n, const int *__restrict__ act) {
+  const size_t w = act[blockIdx.y];
+  const double *a = vin + w * n;
+  double *p = vprev + w * n, *d = dvin + w * n;
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+    const double v = a[e];
+    d[e] = v - p[e];
+    p[e] = v;
+  }
+}
+
+// hacc = hqp (set != 0, a full evaluation) or hacc += hqp (an increment)
+__global__ void k_hacc(double *__restrict__ hacc, const double *__restrict__ hqp, long n, int set,
+                       const int *__restrict__ act) {
+  const size_t w = act[blockIdx.y];
+  double *a = hacc + w * n;
+  const double *b = hqp + w * n;
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x)
+    a[e] = set ? b[e] : a[e] + b[e];
+}
+
+// vprev = vin (the full evaluation just used vin)
+__global__ void k_copy_act(const double *__restrict__ src, double *__restrict__ dst, long n,
+                           const int *__restrict__ act) {
+  const size_t w = act[blockIdx.y];
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x)
+    dst[w * n + e] = src[w * n + e];
 }
 
 // ---- Broyden (pnfam_broyden.f90:117-216), per frequency, FP64 ---------------
@@ -1233,16 +1279,32 @@ struct pnfam_gpu_ctx {
   int hist_cap = -1;
   DevPtr d_vin, d_vout, d_rsp, d_hsp, d_hqp, d_dens, d_flds, d_part, d_cpart, d_S, d_si, d_scal;
   DevPtr d_df, d_dv, d_gram, d_coef, d_wre, d_wim, d_act;
+  // mixed path: last vin fed to the hamiltonian, its increment, accumulated FP64 H20/H02
+  DevPtr d_act64, d_actlo, d_vprev, d_dvin, d_hacc;
+  // split-BF16 grid stage: one element layout per block structure (the mean-field
+  // set in map f, one pairing set per sector), pair tables and operands
+  struct MixSet {
+    int Kp = 0;
+    int kbeg[5] = {0, 0, 0, 0, 0};
+    int ntp = 0, ntd = 0;
+    DevPtr d_elem, d_sa, d_sb, TPh, TPl, TDh, TDl, d_jobs_dens, d_jobs_proj;
+    DevPtr Bh, Bl, O, Fh, Fl;   // per-batch operands
+    int maxcount = 0;            // largest class
+  };
+  MixSet mix[3];                 // 0: mean field (f); 1: kappa+/Delta+ (sector 0); 2: kappa-/Delta- (sector 1)
+  bool mix_ready = false;
+  int RP = 0, mix_cap = 0;
+  int unit_precision = PNFAM_GPU_FP64;   // grid stage of the per-unit entry points
   int bro_nwg = 0;
   double flops_q2s = 0, flops_s2q = 0;
   // per-kernel event timing (pnfam_gpu_profile)
   bool prof = false;
   struct KStat { double ms = 0; int64_t launches = 0; double units = 0; };
-  KStat kstat[16];
+  KStat kstat[32];
   struct Pending { int kid; hipEvent_t a, b; int units; };
   std::vector<Pending> pending;
   std::vector<hipEvent_t> event_pool;
-  double flops_unit[16] = {0};   // algorithmic FLOP per frequency, per kernel id
+  double flops_unit[32] = {0};   // algorithmic FLOP per frequency, per kernel id
   ~pnfam_gpu_ctx() {
     for (auto &p : pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
     for (auto e : event_pool) (void)hipEventDestroy(e);
@@ -1253,10 +1315,12 @@ struct pnfam_gpu_ctx {
 namespace {
 
 enum KernelId { K_TRIPROD_Q2S, K_DENSITY, K_FIELDS, K_PROJECT, K_TRIPROD_S2Q, K_GREEN, K_BRO_RESID, K_BRO_NORM,
-                K_BRO_DOTS, K_BRO_SOLVE, K_BRO_UPDATE, K_CONTRACT, K_CONTRACT_FIN, K_ZERO, K_NKERNEL };
+                K_BRO_DOTS, K_BRO_SOLVE, K_BRO_UPDATE, K_CONTRACT, K_CONTRACT_FIN, K_ZERO, K_MIXACC,
+                K_MIX_CVT, K_MIX_GEMM_D, K_MIX_DENS, K_MIX_GEMM_P, K_NKERNEL };
 const char *kKernelNames[K_NKERNEL] = {"triprod_qp2sp", "density", "fields", "project", "triprod_sp2qp", "green",
                                        "bro_resid", "bro_norm", "bro_dots", "bro_solve", "bro_update", "contract",
-                                       "contract_fin", "zero_slots"};
+                                       "contract_fin", "zero_slots", "mix_accumulate", "mix_convert",
+                                       "mix_gemm_density", "mix_densities", "mix_gemm_project"};
 
 hipEvent_t pool_event(pnfam_gpu_ctx *h) {
   if (!h->event_pool.empty()) { hipEvent_t e = h->event_pool.back(); h->event_pool.pop_back(); return e; }
@@ -1333,6 +1397,11 @@ int ensure_workspace(pnfam_gpu_ctx *h, int npts, int M) {
   if ((e = h->d_wre.alloc(n * 8))) return e;
   if ((e = h->d_wim.alloc(n * 8))) return e;
   if ((e = h->d_act.alloc(n * 4))) return e;
+  if ((e = h->d_act64.alloc(n * 4))) return e;
+  if ((e = h->d_actlo.alloc(n * 4))) return e;
+  if ((e = h->d_vprev.alloc(n * 4 * nxy * 8))) return e;
+  if ((e = h->d_dvin.alloc(n * 4 * nxy * 8))) return e;
+  if ((e = h->d_hacc.alloc(n * 4 * nxy * 8))) return e;
   h->cap = (int)n;
   h->hist_cap = M;
   return 0;
@@ -1367,6 +1436,21 @@ int stage_qp2sp(pnfam_gpu_ctx *h, const double *vin, int na, const int *act) {
   return 0;
 }
 
+int stage_fields(pnfam_gpu_ctx *h, int na, const int *act) {
+  FieldArgs F{};
+  F.dens = h->d_dens.as<double>(); F.flds = h->d_flds.as<double>();
+  F.crho = h->d_crho.as<double>(); F.cs = h->d_cs.as<double>(); F.cpair = h->d_cpair.as<double>();
+  F.cspair = h->d_cspair.as<double>();
+  const double *c = h->cscal.data();
+  F.cdrho = c[0]; F.ctau = c[1]; F.ctj0 = c[2]; F.ctj1 = c[3]; F.ctj2 = c[4]; F.crdj = c[5];
+  F.cds = c[6]; F.ct = c[7]; F.cj = c[8]; F.cgs = c[9]; F.cf = c[10]; F.csdj = c[11];
+  F.nghl = h->nghl; F.act = act;
+  { Prof pr(h, K_FIELDS, na);
+    hipLaunchKernelGGL(k_fields, dim3((h->nghl + 127) / 128, na, 2), dim3(128), 0, h->stream, F); }
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
 int stage_hamiltonian(pnfam_gpu_ctx *h, int na, const int *act) {
   DensArgs A{};
   A.wf = h->d_wf.as<double>(); A.wfdr = h->d_wfdr.as<double>(); A.wfdp = h->d_wfdp.as<double>();
@@ -1386,16 +1470,7 @@ int stage_hamiltonian(pnfam_gpu_ctx *h, int na, const int *act) {
       hipLaunchKernelGGL(k_density, dim3((h->nghl + 63) / 64, na, 2), dim3(64), 0, h->stream, A);
     else
       hipLaunchKernelGGL(k_density_mfma, dim3((h->nghl + 15) / 16, na, 2), dim3(64), 0, h->stream, A); }
-  FieldArgs F{};
-  F.dens = h->d_dens.as<double>(); F.flds = h->d_flds.as<double>();
-  F.crho = h->d_crho.as<double>(); F.cs = h->d_cs.as<double>(); F.cpair = h->d_cpair.as<double>();
-  F.cspair = h->d_cspair.as<double>();
-  const double *c = h->cscal.data();
-  F.cdrho = c[0]; F.ctau = c[1]; F.ctj0 = c[2]; F.ctj1 = c[3]; F.ctj2 = c[4]; F.crdj = c[5];
-  F.cds = c[6]; F.ct = c[7]; F.cj = c[8]; F.cgs = c[9]; F.cf = c[10]; F.csdj = c[11];
-  F.nghl = h->nghl; F.act = act;
-  { Prof pr(h, K_FIELDS, na);
-    hipLaunchKernelGGL(k_fields, dim3((h->nghl + 127) / 128, na, 2), dim3(128), 0, h->stream, F); }
+  if (int e = stage_fields(h, na, act)) return e;
   { Prof pr(h, K_ZERO, na);
     hipLaunchKernelGGL(k_zero_slots, dim3(64, na), dim3(256), 0, h->stream, h->d_hsp.as<double>(), act, 8, 0, 8, h->nxy); }
   ProjArgs P{};
@@ -1411,6 +1486,187 @@ int stage_hamiltonian(pnfam_gpu_ctx *h, int na, const int *act) {
     Prof pr(h, K_PROJECT, na);
     hipLaunchKernelGGL(k_project_mfma, dim3(h->ntiles2, na), dim3(64), 0, h->stream, P, h->d_tiles2.as<Tile2>());
   }
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+// ---- split-BF16 grid stage (mixed path) ------------------------------------
+bool same_pairs(const BlockMap &a, const BlockMap &b) {
+  const auto pa = pair_list(a), pb = pair_list(b);
+  if (pa.size() != pb.size()) return false;
+  for (size_t i = 0; i < pa.size(); ++i)
+    if (pa[i].x != pb[i].x || pa[i].y != pb[i].y || pa[i].z != pb[i].z) return false;
+  return true;
+}
+
+// Element order, pair tables and GEMM jobs of one block structure.
+int mix_build_set(pnfam_gpu_ctx *h, pnfam_gpu_ctx::MixSet &S, const BlockMap &m, int ntp, int ntd) {
+  std::vector<int> elem, sa, sb;
+  const auto pl = pair_list(m);
+  for (int cls = 0; cls < 4; ++cls) {
+    S.kbeg[cls] = (int)elem.size();
+    const int ca = cls >> 1, cb = cls & 1;
+    for (const auto &p : pl) {
+      const int dx = h->db[p.x], dy = h->db[p.y], nx = h->nsu[p.x], ny = h->nsu[p.y];
+      const int a0 = ca ? nx : 0, a1 = ca ? dx : nx, b0 = cb ? ny : 0, b1 = cb ? dy : ny;
+      for (int b = b0; b < b1; ++b)
+        for (int a = a0; a < a1; ++a) {
+          elem.push_back(p.z + a + b * dx);
+          sa.push_back(h->isstart[p.x] + a);
+          sb.push_back(h->isstart[p.y] + b);
+        }
+    }
+    while (elem.size() % MIX_KALIGN) { elem.push_back(-1); sa.push_back(-1); sb.push_back(-1); }
+  }
+  S.kbeg[4] = (int)elem.size();
+  S.Kp = S.kbeg[4];
+  S.ntp = ntp; S.ntd = ntd;
+  S.maxcount = 0;
+  for (int c = 0; c < 4; ++c) S.maxcount = std::max(S.maxcount, S.kbeg[c + 1] - S.kbeg[c]);
+  int e;
+  if ((e = upload(S.d_elem, elem.data(), elem.size() * 4, h->stream)) ||
+      (e = upload(S.d_sa, sa.data(), sa.size() * 4, h->stream)) ||
+      (e = upload(S.d_sb, sb.data(), sb.size() * 4, h->stream)))
+    return e;
+  const size_t ntab = (size_t)S.Kp * h->RP;
+  if ((e = S.TPh.alloc(ntp * ntab * 2)) || (e = S.TPl.alloc(ntp * ntab * 2)) || (e = S.TDh.alloc(ntd * ntab * 2)) ||
+      (e = S.TDl.alloc(ntd * ntab * 2)))
+    return e;
+  TabArgs T{};
+  T.psi[0] = h->d_wf.as<double>(); T.psi[1] = h->d_wfdr.as<double>(); T.psi[2] = h->d_wfdp.as<double>();
+  T.psi[3] = h->d_wfdz.as<double>(); T.psi[4] = h->d_wfd2.as<double>();
+  T.sa = S.d_sa.as<int>(); T.sb = S.d_sb.as<int>();
+  T.Kp = S.Kp; T.nghl = h->nghl; T.RP = h->RP; T.single = ntp == 1;
+  T.nc = ntp; T.dens = 0; T.hi = S.TPh.as<__bf16>(); T.lo = S.TPl.as<__bf16>();
+  hipLaunchKernelGGL(k_mix_tables, dim3(2048, ntp), dim3(256), 0, h->stream, T);
+  T.nc = ntd; T.dens = 1; T.hi = S.TDh.as<__bf16>(); T.lo = S.TDl.as<__bf16>();
+  hipLaunchKernelGGL(k_mix_tables, dim3(2048, ntd), dim3(256), 0, h->stream, T);
+  HIP_TRY(hipGetLastError());
+  // density jobs: table rows (c, r), K = the class range; projection jobs: table
+  // rows = the class range, K = (c, r)
+  GemmJob jd[4], jp[4];
+  for (int c = 0; c < 4; ++c) {
+    jd[c] = GemmJob{0, ntd * h->RP, S.kbeg[c], S.kbeg[c + 1] - S.kbeg[c], c, 0};
+    jp[c] = GemmJob{S.kbeg[c], S.kbeg[c + 1] - S.kbeg[c], 0, h->RP, c, 0};
+  }
+  if ((e = upload(S.d_jobs_dens, jd, sizeof(jd), h->stream)) || (e = upload(S.d_jobs_proj, jp, sizeof(jp), h->stream)))
+    return e;
+  return 0;
+}
+
+int mix_prepare(pnfam_gpu_ctx *h) {
+  if (h->mix_ready) return 0;
+  if (!same_pairs(h->rspmap[0], h->fmap) || !same_pairs(h->rspmap[3], h->fmap))
+    return fail(PNFAM_GPU_EINVAL, "mixed path: rho_pn / rho_np structure differs from f");
+  if (!same_pairs(h->rspmap[1], h->hspmap[1]) || !same_pairs(h->rspmap[2], h->hspmap[2]))
+    return fail(PNFAM_GPU_EINVAL, "mixed path: kappa structure differs from Delta");
+  h->RP = (h->nghl + MIX_RALIGN - 1) / MIX_RALIGN * MIX_RALIGN;
+  int e;
+  if ((e = mix_build_set(h, h->mix[0], h->fmap, NPROJ_C, NDENS_C))) return e;
+  if ((e = mix_build_set(h, h->mix[1], h->hspmap[1], 1, 1))) return e;
+  if ((e = mix_build_set(h, h->mix[2], h->hspmap[2], 1, 1))) return e;
+  h->mix_ready = true;
+  h->mix_cap = 0;
+  // MFMA-issued FLOP per frequency (three BF16 products per split product,
+  // padded element and grid ranges included)
+  double fd = 0, fp = 0;
+  for (int si = 0; si < 3; ++si) {
+    const double cols = si == 0 ? 4.0 : 2.0;
+    fd += 3.0 * 2.0 * h->mix[si].ntd * (double)h->RP * h->mix[si].Kp * cols;
+    fp += 3.0 * 2.0 * h->mix[si].ntp * (double)h->RP * h->mix[si].Kp * cols;
+  }
+  h->flops_unit[K_MIX_GEMM_D] = fd;
+  h->flops_unit[K_MIX_GEMM_P] = fp;
+  return 0;
+}
+
+int mix_workspace(pnfam_gpu_ctx *h, int cap) {
+  if (cap <= h->mix_cap) return 0;
+  const size_t RP = h->RP;
+  for (int si = 0; si < 3; ++si) {
+    auto &S = h->mix[si];
+    const size_t ncol = (si == 0 ? 4 : 2) * (size_t)cap;
+    const size_t nF = (si == 0 ? 4 * (size_t)NPROJ_C : 4) * ncol * RP;
+    int e;
+    if ((e = S.Bh.alloc(ncol * S.Kp * 2)) || (e = S.Bl.alloc(ncol * S.Kp * 2)) ||
+        (e = S.O.alloc(4 * (size_t)S.ntd * RP * ncol * 4)) || (e = S.Fh.alloc(nF * 2)) || (e = S.Fl.alloc(nF * 2)))
+      return e;
+  }
+  h->mix_cap = cap;
+  return 0;
+}
+
+void mix_gemm(pnfam_gpu_ctx *h, GemmArgs G, const pnfam_gpu_ctx::MixSet &S, int tmax) {
+  const int ntt = (tmax + GT_ROWS - 1) / GT_ROWS;
+  const int ntt8 = (ntt + 7) / 8 * 8;
+  G.nst = (G.nS + GS_ROWS - 1) / GS_ROWS;
+  hipLaunchKernelGGL(k_mix_gemm, dim3((unsigned)(ntt8 * G.nst), 4), dim3(512), 0, h->stream, G);
+  (void)S;
+}
+
+// calc_hamiltonian for the active columns on split-BF16 MFMA: rsp -> hsp.
+int stage_hamiltonian_mixed(pnfam_gpu_ctx *h, int na, const int *act) {
+  if (int e = mix_workspace(h, h->cap)) return e;
+  const size_t RP = h->RP;
+  const int cap = h->mix_cap;
+  // 1. rho and kappa -> BF16 hi/lo columns
+  { Prof pr(h, K_MIX_CVT, na);
+    for (int si = 0; si < 3; ++si) {
+      auto &S = h->mix[si];
+      CvtRhoArgs C{};
+      C.rsp = h->d_rsp.as<double>(); C.elem = S.d_elem.as<int>(); C.act = act; C.nxy = h->nxy; C.Kp = S.Kp;
+      C.nsec = si == 0 ? 2 : 1;
+      C.q0 = si == 0 ? 0 : (si == 1 ? 1 : 2);   // quadrants: rho_pn, kappa+, kappa-, rho_np
+      C.q1 = 3;
+      C.hi = S.Bh.as<__bf16>(); C.lo = S.Bl.as<__bf16>();
+      hipLaunchKernelGGL(k_mix_rho, dim3((unsigned)std::min(1024, (S.Kp + 255) / 256), na), dim3(256), 0, h->stream, C);
+    } }
+  // 2. density contractions
+  { Prof pr(h, K_MIX_GEMM_D, na);
+    for (int si = 0; si < 3; ++si) {
+      auto &S = h->mix[si];
+      const long ncol = (si == 0 ? 4 : 2) * (long)cap;
+      GemmArgs G{};
+      G.Th = S.TDh.as<__bf16>(); G.Tl = S.TDl.as<__bf16>(); G.ldT = S.Kp; G.segT = 0;
+      G.Sh = S.Bh.as<__bf16>(); G.Sl = S.Bl.as<__bf16>(); G.ldS = S.Kp; G.segS = 0;
+      G.nseg = 1; G.nS = (si == 0 ? 4 : 2) * na; G.s_cls = 0;
+      G.o_cls = (long)S.ntd * RP * ncol; G.jobs = S.d_jobs_dens.as<GemmJob>();
+      G.mode = 0; G.O = S.O.as<float>(); G.ldO = ncol;
+      mix_gemm(h, G, S, S.ntd * (int)RP);
+    } }
+  // 3. densities
+  { Prof pr(h, K_MIX_DENS, na);
+    CombArgs A{};
+    A.Of = h->mix[0].O.as<float>(); A.Ok0 = h->mix[1].O.as<float>(); A.Ok1 = h->mix[2].O.as<float>();
+    A.ldf = 4L * cap; A.ldk = 2L * cap; A.RP = (int)RP; A.nghl = h->nghl; A.na = na;
+    A.wdcori = h->d_wdcori.as<double>(); A.dens = h->d_dens.as<double>(); A.act = act;
+    hipLaunchKernelGGL(k_mix_dens, dim3(h->nghl, (2 * na + 127) / 128), dim3(128), 0, h->stream, A); }
+  // 4. fields (FP64, closed forms)
+  if (int e = stage_fields(h, na, act)) return e;
+  // 5. fields -> BF16 hi/lo
+  { Prof pr(h, K_MIX_CVT, na);
+    for (int si = 0; si < 3; ++si) {
+      auto &S = h->mix[si];
+      CvtFldArgs C{};
+      C.flds = h->d_flds.as<double>(); C.act = act; C.nghl = h->nghl; C.RP = (int)RP;
+      C.ncap = (si == 0 ? 4 : 2) * cap; C.pair = si != 0; C.sec = si == 2 ? 1 : 0;
+      C.hi = S.Fh.as<__bf16>(); C.lo = S.Fl.as<__bf16>();
+      hipLaunchKernelGGL(k_mix_fields, dim3((unsigned)((RP + 127) / 128), na), dim3(128), 0, h->stream, C);
+    } }
+  // 6. projections straight into the FP64 block matrices
+  { Prof pr(h, K_MIX_GEMM_P, na);
+    for (int si = 0; si < 3; ++si) {
+      auto &S = h->mix[si];
+      const long ncol = (si == 0 ? 4 : 2) * (long)cap;
+      GemmArgs G{};
+      G.Th = S.TPh.as<__bf16>(); G.Tl = S.TPl.as<__bf16>(); G.ldT = RP; G.segT = (long)S.Kp * RP;
+      G.Sh = S.Fh.as<__bf16>(); G.Sl = S.Fl.as<__bf16>(); G.ldS = RP; G.segS = 4 * ncol * (long)RP;
+      G.nseg = S.ntp; G.nS = (si == 0 ? 4 : 2) * na; G.s_cls = ncol * (long)RP; G.o_cls = 0;
+      G.jobs = S.d_jobs_proj.as<GemmJob>();
+      G.mode = 1; G.hsp = h->d_hsp.as<double>(); G.elem = S.d_elem.as<int>(); G.act = act; G.nxy = h->nxy;
+      G.nsec = si == 0 ? 2 : 1; G.q0 = si == 0 ? 0 : (si == 1 ? 1 : 2); G.q1 = 3;
+      mix_gemm(h, G, S, S.maxcount);
+    } }
   HIP_TRY(hipGetLastError());
   return 0;
 }
@@ -1518,6 +1774,7 @@ int pnfam_gpu_set_operator(pnfam_gpu_ctx *h, int32_t beta_minus, const int32_t *
     if (rows[i] >= 0 && seen[rows[i]]++) return fail(PNFAM_GPU_EINVAL, "f has two blocks in one block column");
   }
   h->have_op = false;
+  h->mix_ready = false;
   h->bminus = beta_minus != 0;
   const std::vector<int> &db = h->db;
   std::vector<int> diag(nb), tr(nb);
@@ -1824,11 +2081,25 @@ static int upload_args(pnfam_gpu_ctx *h, DevPtr &dst, const double *in) {
   return 0;
 }
 
+static int unit_hamiltonian(pnfam_gpu_ctx *h) {
+  if (h->unit_precision == PNFAM_GPU_FP64) return stage_hamiltonian(h, 1, h->d_act.as<int>());
+  if (int e = mix_prepare(h)) return e;
+  return stage_hamiltonian_mixed(h, 1, h->d_act.as<int>());
+}
+
+int pnfam_gpu_set_unit_precision(pnfam_gpu_ctx *h, int32_t precision) {
+  if (!h) return fail(PNFAM_GPU_EINVAL, "null argument");
+  if (precision != PNFAM_GPU_FP64 && precision != PNFAM_GPU_MIXED)
+    return fail(PNFAM_GPU_EINVAL, "unit precision must be FP64 or MIXED");
+  h->unit_precision = precision;
+  return 0;
+}
+
 int pnfam_gpu_calc_hamiltonian(pnfam_gpu_ctx *h, const double *in, double *out) {
   if (int e = one_point(h)) return e;
   const long n = h->nxy;
   if (int e = upload_args(h, h->d_rsp, in)) return e;
-  if (int e = stage_hamiltonian(h, 1, h->d_act.as<int>())) return e;
+  if (int e = unit_hamiltonian(h)) return e;
   for (int k = 0; k < 8; ++k)
     HIP_TRY(hipMemcpyAsync(out + (size_t)k * n, h->d_hsp.as<double>() + (size_t)kArgSlot[k] * n, n * 8,
                            hipMemcpyDeviceToHost, h->stream));
@@ -1840,7 +2111,7 @@ int pnfam_gpu_densities(pnfam_gpu_ctx *h, const double *in, int32_t sector, doub
   if (sector != 0 && sector != 1) return fail(PNFAM_GPU_EINVAL, "sector must be 0 or 1");
   if (int e = one_point(h)) return e;
   if (int e = upload_args(h, h->d_rsp, in)) return e;
-  if (int e = stage_hamiltonian(h, 1, h->d_act.as<int>())) return e;
+  if (int e = unit_hamiltonian(h)) return e;
   HIP_TRY(hipMemcpyAsync(dens, h->d_dens.as<double>() + (size_t)sector * 2 * NDENS * h->nghl,
                          (size_t)2 * NDENS * h->nghl * 8, hipMemcpyDeviceToHost, h->stream));
   HIP_TRY(hipStreamSynchronize(h->stream));
@@ -1869,7 +2140,12 @@ int pnfam_gpu_solve_batch(pnfam_gpu_ctx *h, int32_t npts, const double *re_w, co
   if (!h->have_op) return fail(PNFAM_GPU_ESTATE, "no operator set");
   if (npts <= 0) return fail(PNFAM_GPU_EINVAL, "npts must be positive");
   if (o->max_iter < 0) return fail(PNFAM_GPU_EINVAL, "max_iter must be >= 0");
-  if (o->precision != PNFAM_GPU_FP64) return fail(PNFAM_GPU_EINVAL, "only the FP64 path is built");
+  if (o->precision != PNFAM_GPU_FP64 && o->precision != PNFAM_GPU_MIXED && o->precision != PNFAM_GPU_MIXED_NOREFINE)
+    return fail(PNFAM_GPU_EINVAL, "unknown precision");
+  const bool mixed = o->precision != PNFAM_GPU_FP64;
+  const bool refine = o->precision == PNFAM_GPU_MIXED;
+  if (mixed)
+    if (int e = mix_prepare(h)) return e;
   int M = o->broyden_history;
   const bool skip_res = std::fabs(o->quench) < 1e-10;
   if (skip_res) M = -1;   // no residual interaction: no mixing (pnfam_solver.f90:261-264)
@@ -1886,32 +2162,73 @@ int pnfam_gpu_solve_batch(pnfam_gpu_ctx *h, int32_t npts, const double *re_w, co
   const double *e1 = h->bminus ? h->d_Ep.as<double>() : h->d_En.as<double>();
   const double *e2 = h->bminus ? h->d_En.as<double>() : h->d_Ep.as<double>();
   std::vector<double> si(npts, 1.0), S(2 * npts, 0.0);
-  std::vector<int> done(npts, 0), act;
+  std::vector<int> done(npts, 0), refined(npts, 0), act, act64, actlo;
   for (int w = 0; w < npts; ++w) { iters[w] = -1; re_s[w] = 0.0; im_s[w] = 0.0; }
+  if (mixed && !skip_res)
+    hipLaunchKernelGGL(k_fill, grid1((size_t)npts * N, 256), dim3(256), 0, st, h->d_vprev.as<double>(),
+                       (size_t)npts * N, 0.0);
   const int nwg = h->bro_nwg;
+  const long nq = 4 * nxy;   // doubles of H20/H02 (re, im) per frequency
   for (int iter = 0; iter <= o->max_iter; ++iter) {
-    act.clear();
+    act.clear(); act64.clear(); actlo.clear();
     for (int w = 0; w < npts; ++w) {
       if (done[w]) continue;
-      if (si[w] < o->convergence_epsilon) { done[w] = 1; iters[w] = iter; continue; }
+      if (si[w] < o->convergence_epsilon) {
+        if (mixed && refine && !skip_res && !refined[w] && iter < o->max_iter) {
+          // converged on the accumulated field: one full FP64 evaluation replaces
+          // it, and the point iterates on until it meets the tolerance again
+          refined[w] = 1;
+          act.push_back(w); act64.push_back(w);
+          continue;
+        }
+        done[w] = 1; iters[w] = iter; continue;
+      }
       if (iter == o->max_iter) { done[w] = 1; iters[w] = -1; continue; }
       act.push_back(w);
+      (mixed ? actlo : act64).push_back(w);
     }
     if (act.empty()) break;
     const int na = (int)act.size();
     int *dact = h->d_act.as<int>();
     HIP_TRY(hipMemcpyAsync(dact, act.data(), na * 4, hipMemcpyHostToDevice, st));
-    if (!skip_res) {
-      if (int e = stage_qp2sp(h, h->d_vin.as<double>(), na, dact)) return e;
-      if (int e = stage_hamiltonian(h, na, dact)) return e;
-      if (int e = stage_sp2qp(h, na, dact)) return e;
+    // vin = 0 at iteration 0, so H(vin) = 0 exactly there and the chain is not run
+    const bool use_h = !skip_res && iter > 0;
+    if (use_h && !act64.empty()) {
+      const int n64 = (int)act64.size();
+      int *d64 = h->d_act64.as<int>();
+      HIP_TRY(hipMemcpyAsync(d64, act64.data(), n64 * 4, hipMemcpyHostToDevice, st));
+      if (int e = stage_qp2sp(h, h->d_vin.as<double>(), n64, d64)) return e;
+      if (int e = stage_hamiltonian(h, n64, d64)) return e;
+      if (int e = stage_sp2qp(h, n64, d64)) return e;
+      if (mixed) {
+        Prof pr(h, K_MIXACC, n64);
+        hipLaunchKernelGGL(k_hacc, dim3(256, n64), dim3(256), 0, st, h->d_hacc.as<double>(), h->d_hqp.as<double>(),
+                           nq, 1, d64);
+        hipLaunchKernelGGL(k_copy_act, dim3(256, n64), dim3(256), 0, st, h->d_vin.as<double>(),
+                           h->d_vprev.as<double>(), N, d64);
+      }
+    }
+    if (use_h && !actlo.empty()) {
+      const int nlo = (int)actlo.size();
+      int *dlo = h->d_actlo.as<int>();
+      HIP_TRY(hipMemcpyAsync(dlo, actlo.data(), nlo * 4, hipMemcpyHostToDevice, st));
+      { Prof pr(h, K_MIXACC, nlo);
+        hipLaunchKernelGGL(k_delta, dim3(256, nlo), dim3(256), 0, st, h->d_vin.as<double>(), h->d_vprev.as<double>(),
+                           h->d_dvin.as<double>(), N, dlo); }
+      if (int e = stage_qp2sp(h, h->d_dvin.as<double>(), nlo, dlo)) return e;
+      if (int e = (h->basic_kernels ? stage_hamiltonian(h, nlo, dlo) : stage_hamiltonian_mixed(h, nlo, dlo))) return e;
+      if (int e = stage_sp2qp(h, nlo, dlo)) return e;
+      { Prof pr(h, K_MIXACC, nlo);
+        hipLaunchKernelGGL(k_hacc, dim3(256, nlo), dim3(256), 0, st, h->d_hacc.as<double>(), h->d_hqp.as<double>(),
+                           nq, iter == 1 ? 1 : 0, dlo); }
     }
     GreenArgs G{};
-    G.hqp = h->d_hqp.as<double>(); G.f20 = h->d_f20.as<double>(); G.f02 = h->d_f02.as<double>();
+    G.hqp = mixed ? h->d_hacc.as<double>() : h->d_hqp.as<double>();
+    G.f20 = h->d_f20.as<double>(); G.f02 = h->d_f02.as<double>();
     G.e1 = e1; G.e2 = e2;
     G.xrow = h->d_xrow.as<int>(); G.xcol = h->d_xcol.as<int>(); G.yrow = h->d_yrow.as<int>(); G.ycol = h->d_ycol.as<int>();
     G.wre = h->d_wre.as<double>(); G.wim = h->d_wim.as<double>(); G.vout = h->d_vout.as<double>();
-    G.nxy = nxy; G.quench = o->quench; G.use_h = !skip_res; G.act = dact;
+    G.nxy = nxy; G.quench = o->quench; G.use_h = use_h; G.act = dact;
     { Prof pr(h, K_GREEN, na);
       hipLaunchKernelGGL(k_green, dim3((unsigned)std::min<long>(1024, (2 * nxy + 255) / 256), na), dim3(256), 0, st, G); }
     // Broyden

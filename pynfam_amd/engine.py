@@ -41,6 +41,7 @@ class _Opts(C.Structure):
 
 FP64 = 0
 MIXED = 1
+MIXED_NOREFINE = 2
 
 
 class GpuError(RuntimeError):
@@ -68,6 +69,7 @@ def load_library(path=LIB_PATH):
     lib.pnfam_gpu_sp2qp.argtypes = [P, _dp, _dp, _dp, _dp, _dp]
     lib.pnfam_gpu_densities.argtypes = [P, _dp, C.c_int32, _dp]
     lib.pnfam_gpu_profile.argtypes = [P, C.c_int32]
+    lib.pnfam_gpu_set_unit_precision.argtypes = [P, C.c_int32]
     lib.pnfam_gpu_profile_read.argtypes = [P, C.c_int32, C.c_char_p, _dp, C.POINTER(C.c_int64), _dp, _dp,
                                            C.POINTER(C.c_int32)]
     lib.pnfam_gpu_destroy.argtypes = [P]
@@ -76,7 +78,8 @@ def load_library(path=LIB_PATH):
     lib.pnfam_gpu_version.restype = C.c_char_p
     for name in ("pnfam_gpu_create", "pnfam_gpu_set_operator", "pnfam_gpu_solve_batch", "pnfam_gpu_structures",
                  "pnfam_gpu_get_fqp", "pnfam_gpu_qp2sp", "pnfam_gpu_calc_hamiltonian", "pnfam_gpu_sp2qp",
-                 "pnfam_gpu_densities", "pnfam_gpu_profile", "pnfam_gpu_profile_read"):
+                 "pnfam_gpu_densities", "pnfam_gpu_profile", "pnfam_gpu_profile_read",
+                 "pnfam_gpu_set_unit_precision"):
         getattr(lib, name).restype = C.c_int
     _lib = lib
     return lib
@@ -177,6 +180,10 @@ class GpuEngine:
         return out
 
     # ---- per-unit entry points ---------------------------------------------
+    def set_unit_precision(self, precision):
+        """Grid stage used by calc_hamiltonian() and densities(): FP64 or MIXED."""
+        self._check(self.lib.pnfam_gpu_set_unit_precision(self.h, int(precision)))
+
     def structures(self):
         nb = self.prob.basis.nb
         maps = np.zeros((4, nb), dtype=np.int32)

@@ -160,11 +160,71 @@ def extract_set(name, rel):
         print(name, op, len(rows), "points")
 
 
+def _table(path):
+    """Whitespace table of a pynfam .out file: header line after the '#' block,
+    then one row per point (index column first)."""
+    with open(path) as fh:
+        lines = [l for l in fh.read().splitlines() if l.strip()]
+    head = [l for l in lines if l.startswith("#")]
+    body = [l for l in lines if not l.startswith("#")]
+    cols = body[0].split()
+    rows = []
+    for l in body[1:]:
+        parts = l.split()
+        vals = []
+        for x in parts[1:]:
+            try:
+                vals.append(float(x))
+            except ValueError:
+                vals.append(x)
+        rows.append(vals)
+    return head, cols, rows
+
+
+def _summary(head):
+    """Key = value pairs of the '# Summary Data' block."""
+    out = {}
+    for l in head:
+        for k, v in re.findall(r"([\w|/*()]+(?:\s\w+)?)\s*=\s*(\S+)", l):
+            try:
+                out[k.strip()] = float(v)
+            except ValueError:
+                out[k.strip()] = v
+    return out
+
+
+def extract_rates(name, rel):
+    """Contour tables (GT-K*.out), phase space on the contour and the rates of
+    the archived beta-decay post-processing (pynfam/strength), as data."""
+    src = os.path.join(REF, rel)
+    out = {}
+    for op in ("GT-K0", "GT-K1"):
+        _, cols, rows = _table(os.path.join(src, "fam_soln", op + ".out"))
+        out[op] = {"columns": cols, "rows": rows}
+    meta = os.path.join(src, "beta_soln", "beta_meta")
+    for part in ("re", "im"):
+        head, cols, rows = _table(os.path.join(meta, "phasespace_%s.out" % part))
+        out["phasespace_" + part] = {"columns": cols, "rows": rows}
+    head, cols, rows = _table(os.path.join(src, "beta_soln", "beta.out"))
+    out["summary"] = _summary(head)
+    with open(os.path.join(src, "beta_soln", "beta.out")) as fh:
+        rates = {}
+        for l in fh.read().splitlines():
+            m = re.match(r"(\S+)\s+(" + FLOAT + r")\s+(\S+)$", l)
+            if m and not l.startswith("#"):
+                rates[m.group(1)] = _f(m.group(2))
+    out["rates"] = rates
+    with open(os.path.join(HERE, name, "rates.json"), "w") as fh:
+        json.dump(out, fh, indent=0)
+    print(name, "rates", sorted(rates))
+
+
 def main():
     if not os.path.isdir(REF):
         sys.exit("reference tree not present")
     for name, rel in SETS.items():
         extract_set(name, rel)
+        extract_rates(name, rel)
 
 
 if __name__ == "__main__":

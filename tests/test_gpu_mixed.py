@@ -1,0 +1,89 @@
+"""Mixed-precision path against the reference's FP64 runs.
+
+The residual interaction is linear in the amplitudes, so the mixed path carries
+H(v_k) as an FP64 sum of H applied to the increments v_k - v_{k-1}; the
+increments go through the grid stage in reduced precision.  Tolerances, written
+here per mode:
+  MIXED (one FP64 refinement evaluation at first convergence): 1e-6 relative in S;
+  MIXED_NOREFINE: 1e-4 relative in S.
+Both must converge every archived point."""
+import numpy as np
+import pytest
+
+from conftest import golden_points, golden_problem
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"MIXED": 1e-6, "MIXED_NOREFINE": 1e-4}
+
+
+@pytest.mark.parametrize("mode", ["MIXED", "MIXED_NOREFINE"])
+@pytest.mark.parametrize("nucleus,K", [("S40", 0), ("S40", 1), ("Gd162", 0), ("Gd162", 1)])
+def test_mixed_matches_reference_runs(nucleus, K, mode):
+    from pynfam_amd import engine as E
+    prob = golden_problem(nucleus, K)
+    eng = E.GpuEngine(prob)
+    pts = golden_points(nucleus, K)
+    w = np.array([complex(*p["energy"]) for p in pts])
+    S = np.array([complex(*p["strength"]) for p in pts])
+    r = eng.solve(w, precision=getattr(E, mode))
+    assert np.all(r["iterations"] > 0), r["iterations"]
+    err = np.abs(r["strength"] - S) / np.abs(S)
+    print(f"{nucleus} K={K} {mode}: max rel err {err.max():.2e}, iterations {r['iterations'].tolist()}")
+    assert err.max() < TOL[mode], err.max()
+    eng.close()
+
+
+@pytest.mark.parametrize("K", [0, 1])
+def test_mixed_grid_stage_against_fp64(K):
+    """One calc_hamiltonian through the split-BF16 pair-table contractions
+    against the FP64 grid stage on the same random densities: every output
+    array within 1e-4 of its largest element."""
+    from oracle.oracle import Oracle
+    from pynfam_amd import engine as E
+    prob = golden_problem("S40", K)
+    eng = E.GpuEngine(prob)
+    orc = Oracle(prob)
+    rng = np.random.default_rng(40 + K)
+    rsp, _ = orc.qp_to_sp(*(rng.standard_normal(prob.nxy) for _ in range(4)))
+    args = rsp[[0, 1, 2, 3, 6, 7, 4, 5]]
+    ref = eng.calc_hamiltonian(args)
+    dref = [eng.densities(args, s) for s in (0, 1)]
+    eng.set_unit_precision(E.MIXED)
+    got = eng.calc_hamiltonian(args)
+    dgot = [eng.densities(args, s) for s in (0, 1)]
+    eng.set_unit_precision(E.FP64)
+    for s in (0, 1):
+        top = np.max(np.abs(dref[s]))
+        for d in range(56):
+            # densities that vanish by symmetry come out at round-off of the sector's scale
+            scale = max(np.max(np.abs(dref[s][d])), 1e-3 * top)
+            assert np.max(np.abs(dgot[s][d] - dref[s][d])) < 1e-4 * scale, (s, d)
+    for k in range(8):
+        scale = np.max(np.abs(ref[k]))
+        err = np.max(np.abs(got[k] - ref[k])) / scale
+        print(f"K={K} arg {k}: max err / max {err:.2e}")
+        assert err < 1e-4, (k, err)
+    eng.close()
+
+
+def test_mixed_edge_cases():
+    """Interrupted points, no residual interaction and Broyden history 0 behave
+    as on the FP64 path."""
+    from pynfam_amd import engine as E
+    prob = golden_problem("S40", 0)
+    eng = E.GpuEngine(prob)
+    w = np.array([0.5 + 0.3j, 12.0 + 1.0j])
+    ref = eng.solve(w)
+    for mode in (E.MIXED, E.MIXED_NOREFINE):
+        r = eng.solve(w[:1], max_iter=3, precision=mode)
+        assert r["iterations"][0] == -1
+        r = eng.solve(w, quench=0.0, precision=mode)
+        r64 = eng.solve(w, quench=0.0)
+        assert np.all(r["iterations"] == 2)
+        assert np.max(np.abs(r["strength"] - r64["strength"])) == 0.0
+        # linear mixing converges slowly; the refinement restarts it once
+        r = eng.solve(w[:1], broyden_history=0, max_iter=150, precision=mode)
+        assert r["iterations"][0] > 0
+        assert abs(r["strength"][0] - ref["strength"][0]) < 1e-4 * abs(ref["strength"][0])
+    eng.close()
