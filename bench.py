@@ -24,16 +24,15 @@ sys.path.insert(0, ROOT)
 METRIC = "converged FAM freq points/sec at N_sh=16 (mixed vs FP64 GPU), rate err vs FP64"
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
+Z_PARENT, N_PARENT = 60, 92
 
 
-def contour(npts, center=8.0, radius=10.0):
-    """Upper half of a circle through [center-radius, center+radius] with
-    Gauss-Legendre nodes in the angle (the closed-contour half that is computed,
-    the other half follows from S(w*) = S(w)*)."""
-    import numpy as np
-    x, _ = np.polynomial.legendre.leggauss(2 * npts)
-    th = np.pi * (x[x > 0] + 0.0)                      # (0, pi)
-    return center + radius * np.exp(1j * th)
+def contour(npts, eqrpa_max):
+    """The computed half (npts points) of pynfam's CIRCLE contour on
+    [0, EQRPA_max] with 2*npts Gauss-Legendre angle nodes; the other half
+    follows from S(w*) = S(w)* (pynfam/strength/contour.py:212-281)."""
+    from pynfam_amd.rate import circle_contour
+    return circle_contour(0.0, eqrpa_max, 2 * npts)["z"][:npts]
 
 
 def build_workload(nsh, beta, Z, N, ngrid):
@@ -137,10 +136,17 @@ def measure(engines, w, precision, args):
             "strength": S}
 
 
-def rate_error(w, s_ref, s_mix, nk):
-    """Relative error of the contour-integrated rate when the mixed strengths
-    replace the FP64 ones (None until a phase space is supplied)."""
-    return None
+def rate_error(emax, npts, s_ref, s_mix):
+    """Allowed-GT beta-minus rates (K=0, K=1 and total) of the daughter
+    Z+1, A from the FP64 and the mixed strengths on the same contour and phase
+    space; relative differences."""
+    from pynfam_amd.rate import gt_rates_from_half
+    zd, a = Z_PARENT + 1, Z_PARENT + N_PARENT
+    r64, _, _ = gt_rates_from_half(zd, a, emax, 0.0, 2 * npts, {0: s_ref[:npts], 1: s_ref[npts:]})
+    rmx, _, _ = gt_rates_from_half(zd, a, emax, 0.0, 2 * npts, {0: s_mix[:npts], 1: s_mix[npts:]})
+    return {"total": abs(rmx["total"] - r64["total"]) / abs(r64["total"]),
+            "K0": abs(rmx[0] - r64[0]) / abs(r64[0]), "K1": abs(rmx[1] - r64[1]) / abs(r64[1]),
+            "rate_fp64_per_s": r64["total"], "eqrpa_max_MeV": emax}
 
 
 def main():
@@ -171,9 +177,11 @@ def main():
     import numpy as np
     from pynfam_amd import engine as E
 
-    basis, probs = build_workload(args.nsh, args.beta, 60, 92, args.ngrid)
+    from pynfam_amd.rate import eqrpa_max_beta_minus
+    basis, probs = build_workload(args.nsh, args.beta, Z_PARENT, N_PARENT, args.ngrid)
+    emax = eqrpa_max_beta_minus(basis.sols[0].ala, basis.sols[1].ala)
     engines = [E.GpuEngine(p) for p in probs]
-    w = contour(args.npts)
+    w = contour(args.npts, emax)
     modes = {"fp64": [E.FP64], "mixed": [E.MIXED_NOREFINE], "mixed_refine": [E.MIXED],
              "both": [E.FP64, E.MIXED_NOREFINE]}[args.precision]
     runs = {}
@@ -212,7 +220,7 @@ def main():
                        "mean_iterations": ref["mean_iterations"], "kernel_ms": ref["kernel_ms"]}
         out["mixed_vs_fp64"] = mix["value"] / ref["value"]
         out["strength_rel_err_vs_fp64"] = {"max": float(err.max()), "median": float(np.median(err))}
-        out["rate_rel_err_vs_fp64"] = rate_error(w, ref["strength"], mix["strength"], len(engines))
+        out["rate_rel_err_vs_fp64"] = rate_error(emax, len(w), ref["strength"], mix["strength"])
     print(json.dumps(out), flush=True)
 
 

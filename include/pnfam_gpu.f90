@@ -1,0 +1,169 @@
+!------------------------------------------------------------------------------
+! Fortran binding of the pnfam_gpu C ABI (pnfam_gpu.h) through ISO_C_BINDING.
+!
+! A Fortran host (the reference's pnfam_solve, exes/pnfam/pnfam_solver.f90:32-221)
+! uses it to hand its resident HFB data to the MI355X engine once and to solve
+! frequencies in batches.  Arrays pass as the reference stores them (blockmatrix
+! elem arrays, column-major wf tables); maps are 0-based, -1 = no block, so a
+! caller subtracts 1 from the reference's 1-based ir2c.
+!------------------------------------------------------------------------------
+module pnfam_gpu
+   use iso_c_binding
+   implicit none
+
+   integer(c_int32_t), parameter :: PNFAM_GPU_OK = 0, PNFAM_GPU_EINVAL = -1, PNFAM_GPU_EDEVICE = -2, &
+                                    PNFAM_GPU_ESTATE = -3, PNFAM_GPU_ENOMEM = -4
+   integer(c_int32_t), parameter :: PNFAM_GPU_FP64 = 0, PNFAM_GPU_MIXED = 1, PNFAM_GPU_MIXED_NOREFINE = 2
+
+   ! pnfam_gpu_basis: omega-independent HFB / functional inputs (pointers to the
+   ! caller's arrays; the library copies them in pnfam_gpu_create)
+   type, bind(C) :: pnfam_gpu_basis
+      integer(c_int32_t) :: nb, dqp, nghl
+      type(c_ptr) :: db, num_spin_up, ns
+      type(c_ptr) :: wf, wfdr, wfdz, wfdp, wfd2_all
+      type(c_ptr) :: wdcori, Ep, En
+      type(c_ptr) :: Up, Vp, Un, Vn
+      type(c_ptr) :: crho, cs, cpair, cspair
+      type(c_ptr) :: cscal
+   end type
+
+   type, bind(C) :: pnfam_gpu_opts
+      integer(c_int32_t) :: max_iter
+      real(c_double) :: convergence_epsilon
+      integer(c_int32_t) :: broyden_history
+      real(c_double) :: quench
+      real(c_double) :: alpha_mix
+      integer(c_int32_t) :: precision
+   end type
+
+   interface
+      integer(c_int) function pnfam_gpu_create(b, h) bind(C, name="pnfam_gpu_create")
+         import :: c_int, c_ptr, pnfam_gpu_basis
+         type(pnfam_gpu_basis), intent(in) :: b
+         type(c_ptr), intent(out) :: h
+      end function
+
+      integer(c_int) function pnfam_gpu_set_operator(h, beta_minus, f_ir2c, f_elem, nxy) &
+            bind(C, name="pnfam_gpu_set_operator")
+         import :: c_int, c_int32_t, c_int64_t, c_ptr, c_double
+         type(c_ptr), value :: h
+         integer(c_int32_t), value :: beta_minus
+         integer(c_int32_t), intent(in) :: f_ir2c(*)
+         real(c_double), intent(in) :: f_elem(*)
+         integer(c_int64_t), intent(out) :: nxy
+      end function
+
+      integer(c_int) function pnfam_gpu_solve_batch(h, npts, re_w, im_w, o, re_s, im_s, iters, si_final, x_out) &
+            bind(C, name="pnfam_gpu_solve_batch")
+         import :: c_int, c_int32_t, c_ptr, c_double, pnfam_gpu_opts
+         type(c_ptr), value :: h
+         integer(c_int32_t), value :: npts
+         real(c_double), intent(in) :: re_w(*), im_w(*)
+         type(pnfam_gpu_opts), intent(in) :: o
+         real(c_double), intent(out) :: re_s(*), im_s(*)
+         integer(c_int32_t), intent(out) :: iters(*)
+         real(c_double), intent(out) :: si_final(*)
+         type(c_ptr), value :: x_out
+      end function
+
+      integer(c_int) function pnfam_gpu_structures(h, maps, xy_maps) bind(C, name="pnfam_gpu_structures")
+         import :: c_int, c_int32_t, c_ptr
+         type(c_ptr), value :: h
+         integer(c_int32_t), intent(out) :: maps(*), xy_maps(*)
+      end function
+
+      integer(c_int) function pnfam_gpu_get_fqp(h, f20, f02) bind(C, name="pnfam_gpu_get_fqp")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: h
+         real(c_double), intent(out) :: f20(*), f02(*)
+      end function
+
+      integer(c_int) function pnfam_gpu_qp2sp(h, x_re, x_im, y_re, y_im, rsp) bind(C, name="pnfam_gpu_qp2sp")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: h
+         real(c_double), intent(in) :: x_re(*), x_im(*), y_re(*), y_im(*)
+         real(c_double), intent(out) :: rsp(*)
+      end function
+
+      ! the 16 blockmatrix arguments of calc_dHsp_interface (pnfam_setup.f90:56-71)
+      ! as in(8*nxy) and out(8*nxy), elem arrays back to back in argument order
+      integer(c_int) function pnfam_gpu_calc_hamiltonian(h, in, out) bind(C, name="pnfam_gpu_calc_hamiltonian")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: h
+         real(c_double), intent(in) :: in(*)
+         real(c_double), intent(out) :: out(*)
+      end function
+
+      integer(c_int) function pnfam_gpu_sp2qp(h, hsp, h20_re, h20_im, h02_re, h02_im) bind(C, name="pnfam_gpu_sp2qp")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: h
+         real(c_double), intent(in) :: hsp(*)
+         real(c_double), intent(out) :: h20_re(*), h20_im(*), h02_re(*), h02_im(*)
+      end function
+
+      integer(c_int) function pnfam_gpu_set_unit_precision(h, precision) bind(C, name="pnfam_gpu_set_unit_precision")
+         import :: c_int, c_int32_t, c_ptr
+         type(c_ptr), value :: h
+         integer(c_int32_t), value :: precision
+      end function
+
+      integer(c_int) function pnfam_gpu_densities(h, in, sector, dens) bind(C, name="pnfam_gpu_densities")
+         import :: c_int, c_int32_t, c_ptr, c_double
+         type(c_ptr), value :: h
+         real(c_double), intent(in) :: in(*)
+         integer(c_int32_t), value :: sector
+         real(c_double), intent(out) :: dens(*)
+      end function
+
+      integer(c_int) function pnfam_gpu_profile(h, enable) bind(C, name="pnfam_gpu_profile")
+         import :: c_int, c_int32_t, c_ptr
+         type(c_ptr), value :: h
+         integer(c_int32_t), value :: enable
+      end function
+
+      integer(c_int) function pnfam_gpu_profile_read(h, max_kernels, names, ms, launches, units, flops_per_unit, &
+            n_out) bind(C, name="pnfam_gpu_profile_read")
+         import :: c_int, c_int32_t, c_int64_t, c_ptr, c_double, c_char
+         type(c_ptr), value :: h
+         integer(c_int32_t), value :: max_kernels
+         character(kind=c_char), intent(out) :: names(*)
+         real(c_double), intent(out) :: ms(*), units(*), flops_per_unit(*)
+         integer(c_int64_t), intent(out) :: launches(*)
+         integer(c_int32_t), intent(out) :: n_out
+      end function
+
+      subroutine pnfam_gpu_destroy(h) bind(C, name="pnfam_gpu_destroy")
+         import :: c_ptr
+         type(c_ptr), value :: h
+      end subroutine
+
+      type(c_ptr) function pnfam_gpu_last_error() bind(C, name="pnfam_gpu_last_error")
+         import :: c_ptr
+      end function
+
+      type(c_ptr) function pnfam_gpu_version() bind(C, name="pnfam_gpu_version")
+         import :: c_ptr
+      end function
+   end interface
+
+contains
+
+   ! C string (NUL-terminated) to a Fortran character variable
+   function pnfam_gpu_string(p) result(s)
+      type(c_ptr), intent(in) :: p
+      character(len=:), allocatable :: s
+      character(kind=c_char), pointer :: cs(:)
+      integer :: n
+      interface
+         integer(c_size_t) function c_strlen(q) bind(C, name="strlen")
+            import :: c_size_t, c_ptr
+            type(c_ptr), value :: q
+         end function
+      end interface
+      n = int(c_strlen(p))
+      call c_f_pointer(p, cs, [n])
+      allocate(character(len=n) :: s)
+      s = transfer(cs(1:n), s)
+   end function
+
+end module pnfam_gpu

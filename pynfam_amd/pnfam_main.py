@@ -1,0 +1,188 @@
+"""Drop-in for pnfam_main.x: one pnFAM point per process, on the MI355X engine.
+
+The executable contract pynfam relies on (pynfam/fortran/pnfam_run.py:140-166,
+fortran_utils.py:190-258, outputs/pnfam_parser.py:33-132):
+- zero or one command-line argument, the namelist file (default
+  pnfam_NAMELIST.dat, exes/pnfam/pnfam_prog.f90:12-14); more than one argument
+  aborts;
+- the namelist groups of exes/pnfam/pnfam_setup.f90:98-108 with its defaults;
+- the HFB solution from the working directory (hfbtho_output.hel);
+- output <fam_output_filename>.dat (or stdout when the name is empty) with the
+  header, the convergence line "FAM iteration converged/interrupted after N
+  steps", "Total CPU time = X minutes" and the whitespace result table
+  "Real Imag / Energy / Strength" (pnfam_txtoutput.f90:230-253, format
+  (1x,A,2ES34.19));
+- errors: a message in the log and exit status 0, as the reference's abort
+  (pnfam_logger.f90:98-105); pynfam detects them from the missing table.
+
+The iteration runs through the C ABI (pnfam_gpu_solve_batch) with the
+precision chosen by the environment variable PNFAM_GPU_PRECISION (fp64,
+mixed, mixed_norefine; default fp64).
+"""
+from __future__ import annotations
+
+import os
+import sys
+import time
+
+from . import namelist
+
+VERSION = "2.00-mi355x"
+HEL_FILE = "hfbtho_output.hel"
+DEFAULT_NAMELIST = "pnfam_NAMELIST.dat"
+
+
+class Log:
+    """writelog: '#'-prefixed lines to <name>.dat and/or stdout."""
+
+    def __init__(self, path, to_stdout):
+        self.fh = open(path, "w") if path else None
+        self.to_stdout = to_stdout or not path
+
+    def line(self, s="", raw=False):
+        txt = s if raw else "# " + s if s else "#"
+        if self.fh:
+            self.fh.write(txt + "\n")
+            self.fh.flush()
+        if self.to_stdout:
+            print(txt, flush=True)
+
+    def close(self):
+        if self.fh:
+            self.fh.close()
+
+
+def interaction_overrides(inter):
+    """Namelist interaction entries that were set (the reference marks unset
+    ones with +-huge, pnfam_setup.f90:196-209)."""
+    kw = {"require_gauge_invariance": bool(inter["require_gauge_invariance"]),
+          "force_j2_terms": bool(inter["force_j2_terms"])}
+    for k in ("vpair0", "vpair1", "vpair_t0", "vpair_t1"):
+        if inter[k] > -0.5 * namelist.HUGE:
+            kw[k] = float(inter[k])
+    for k in ("override_cs0", "override_csr", "override_cds", "override_ct", "override_cf", "override_cgs",
+              "override_cj", "override_csdj"):
+        if inter[k] < 0.5 * namelist.HUGE:
+            kw[k] = float(inter[k])
+    return kw
+
+
+def build_from_namelist(nml, hel_path):
+    from .problem import build_problem
+    ext, inter, solv = nml["ext_field"], nml["interaction"], nml["solver"]
+    if int(ext["two_body_current_mode"]) != 0:
+        raise ValueError("two-body currents are not supported by this engine")
+    op = str(ext["operator_name"]).strip().upper()
+    if op not in ("F", "GT"):
+        raise ValueError(f"operator {op!r} is not supported by this engine (F, GT)")
+    return build_problem(hel_path, op, int(ext["operator_k"]), str(ext["beta_type"]).strip(),
+                         str(inter["interaction_name"]).strip(),
+                         quench_residual_int=float(solv["quench_residual_int"]),
+                         energy_shift_prot=float(solv["energy_shift_prot"]),
+                         energy_shift_neut=float(solv["energy_shift_neut"]),
+                         **interaction_overrides(inter))
+
+
+def precision_from_env():
+    from . import engine as E
+    name = os.environ.get("PNFAM_GPU_PRECISION", "fp64").strip().lower()
+    table = {"fp64": E.FP64, "mixed": E.MIXED, "mixed_norefine": E.MIXED_NOREFINE}
+    if name not in table:
+        raise ValueError(f"PNFAM_GPU_PRECISION must be one of {sorted(table)}")
+    return table[name], name
+
+
+def write_header(log, nml, prob, parameter_filename, precision_name):
+    g, ext, solv = nml["general"], nml["ext_field"], nml["solver"]
+    b = prob.basis
+    op = f"{prob.operator}{ext['beta_type']}"
+    sep = "=" * 51
+    log.line()
+    log.line(sep)
+    log.line("pnFAM".center(51))
+    log.line("Charge-changing Finite Amplitude Method".center(51))
+    log.line(f"Version: {VERSION}".center(51))
+    log.line(f"MI355X (gfx950) engine, {precision_name} path".center(51))
+    log.line(sep)
+    log.line(f"Operator: {op} for K = {prob.K}".center(51))
+    sn = "-" if g["imag_eqrpa"] < 0 else "+"
+    log.line(f"Energy: ({g['real_eqrpa']:8.4f} {sn} {abs(g['imag_eqrpa']):7.4f}i)".center(51))
+    log.line(sep)
+    log.line()
+    log.line("-" * 51)
+    log.line("pnFAM solver details".center(51))
+    log.line("-" * 51)
+    log.line(f"FAM input parameter file name:   '{parameter_filename}'")
+    log.line(f"FAM output file name base:       '{g['fam_output_filename']}'")
+    log.line()
+    log.line(f"Operator:                        {op}  K= {prob.K}")
+    log.line(f"Re(EQRPA):                       {g['real_eqrpa']:8.4f} MeV")
+    log.line(f"Im(EQRPA):                       {g['imag_eqrpa']:8.4f} MeV")
+    log.line()
+    log.line(f"Number shells:                   {int(b.hel.n00)}")
+    log.line(f"Basis size:                      {b.dqp}")
+    log.line(f"Number matrix blocks:            {b.nb}")
+    log.line(f"Non-trivial FAM matrix elements: {prob.nxy}")
+    log.line()
+    log.line(f"Maximum iterations:              {int(solv['max_iter'])}")
+    log.line(f"Broyden history size:            {int(solv['broyden_history_size'])}")
+    log.line("Broyden mixing factor:           0.70")
+    log.line(f"Convergence limit:               {float(solv['convergence_epsilon']):8.1E}")
+    log.line()
+    log.line(f"Residual interaction: {prob.couplings.name}")
+    log.line()
+
+
+def write_result(log, omega, strength, iterations, si, max_iter, seconds):
+    log.line("-" * 72)
+    if iterations >= 0:
+        log.line(f" *  FAM iteration converged after {iterations:6d} steps.  si = {si:12.5E}")
+    else:
+        log.line(f" *  FAM iteration interrupted after {max_iter:6d} steps.  si = {si:12.5E}")
+    log.line("-" * 72)
+    log.line(f"Total CPU time = {seconds / 60.0:11.3E} minutes")
+    log.line()
+    log.line()
+    log.line("Result (Energy [MeV], strength and cross-terms [MeV^-1]):")
+    log.line("-" * 78)
+    log.line(f"{'Real':>44s}{'Imag':>34s}", raw=True)
+    log.line(f" {'Energy':<10s}{omega.real:34.19E}{omega.imag:34.19E}", raw=True)
+    log.line(f" {'Strength':<10s}{strength.real:34.19E}{strength.imag:34.19E}", raw=True)
+
+
+def run(parameter_filename, workdir="."):
+    t0 = time.process_time()
+    nml = namelist.read(parameter_filename)
+    g, solv = nml["general"], nml["solver"]
+    name = str(g["fam_output_filename"]).strip()
+    log = Log(os.path.join(workdir, name + ".dat") if name else None, bool(g["print_stdout"]))
+    try:
+        from .engine import GpuEngine
+        prec, prec_name = precision_from_env()
+        prob = build_from_namelist(nml, os.path.join(workdir, HEL_FILE))
+        write_header(log, nml, prob, parameter_filename, prec_name)
+        omega = complex(float(g["real_eqrpa"]), float(g["imag_eqrpa"]))
+        eng = GpuEngine(prob)
+        r = eng.solve([omega], max_iter=int(solv["max_iter"]), eps=float(solv["convergence_epsilon"]),
+                      broyden_history=int(solv["broyden_history_size"]), precision=prec)
+        eng.close()
+        write_result(log, omega, complex(r["strength"][0]), int(r["iterations"][0]), float(r["si"][0]),
+                     int(solv["max_iter"]), time.process_time() - t0)
+        return 0
+    except Exception as exc:   # the reference's abort: message, then a normal stop
+        log.line(f" ERROR: {exc}")
+        return 0
+    finally:
+        log.close()
+
+
+def main(argv=None):
+    args = sys.argv[1:] if argv is None else list(argv)
+    if len(args) > 1:
+        print(" More than one command-line argument! Aborting.")
+        return 0
+    return run(args[0] if args else DEFAULT_NAMELIST)
+
+
+if __name__ == "__main__":
+    sys.exit(main())

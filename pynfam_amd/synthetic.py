@@ -14,7 +14,10 @@ used for parity.
   quadrature the reference stores in .hel files);
 * wavefunction tables: the same HO recurrences as real solutions (hfb.py);
 * U, V: seeded random block-orthogonal rotations with BCS u, v;
-  E = sqrt((e - lambda)^2 + Delta^2), zeroed above a 60 MeV pairing window.
+  E = sqrt((e - lambda)^2 + Delta^2), zeroed above a 60 MeV pairing window;
+* Fermi energies lambda_n = -5.5, lambda_p = -8.0 MeV (gaps 1.2, 1.0 MeV): a
+  beta-minus window EQRPA_max = Delta M_nH + lambda_n - lambda_p = 3.28 MeV
+  above the lowest two-quasiparticle energies, as for a neutron-rich emitter.
 """
 from __future__ import annotations
 
@@ -25,6 +28,7 @@ import numpy as np
 from . import hfb
 
 SEED = 20261202
+LAMBDA_N, LAMBDA_P = -5.5, -8.0
 
 
 def hfbtho_blocks(n00, beta, n00max=50):
@@ -80,7 +84,7 @@ def synthetic_basis(Z=60, N=92, n00=16, beta=0.3, ngh=40, ngl=40, seed=SEED) -> 
     hw0 = 41.0 * (Z + N) ** (-1.0 / 3.0)
     sols = []
     ia = np.concatenate([[0], np.cumsum(h.ID)[:-1]])
-    for it, (lam, delta) in enumerate([(-7.0, 1.2), (-8.0, 1.0)]):
+    for it, (lam, delta) in enumerate([(LAMBDA_N, 1.2), (LAMBDA_P, 1.0)]):
         E = np.zeros(h.nt)
         act = np.zeros(h.nt, dtype=bool)
         Us, Vs = [], []

@@ -154,6 +154,10 @@ def extract_set(name, rel):
                     d["namelist"] = parse_nml_text(txt)
                 elif member.name.endswith(".dat"):
                     d.update(parse_dat(txt))
+                if member.name.endswith("000000/" + op + ".in"):
+                    # one raw namelist as written by pynfam, for the input-reader tests
+                    with open(os.path.join(dst, op + "_000000.in"), "w") as fh:
+                        fh.write(txt)
         rows = [dict(point=k, **v) for k, v in sorted(points.items())]
         with open(os.path.join(dst, op + ".json"), "w") as fh:
             json.dump(rows, fh, indent=0)

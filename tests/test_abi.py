@@ -3,6 +3,10 @@ exports every entry point include/pnfam_gpu.h declares."""
 import ctypes
 import os
 import re
+import shutil
+import subprocess
+
+import pytest
 
 from conftest import ROOT
 
@@ -40,3 +44,19 @@ def test_engine_refuses_to_run_without_library(tmp_path, monkeypatch):
     import pytest
     with pytest.raises(eng.GpuError):
         eng.load_library(str(tmp_path / "missing.so"))
+
+
+@pytest.mark.skipif(shutil.which("amdflang") is None, reason="no Fortran compiler")
+def test_fortran_binding_compiles_and_links(tmp_path):
+    """include/pnfam_gpu.f90 (ISO_C_BINDING interfaces) compiles, links against
+    the library and calls it without a device (version, argument check)."""
+    inc = os.path.join(ROOT, "include", "pnfam_gpu.f90")
+    prog = os.path.join(ROOT, "tests", "fortran", "check_binding.f90")
+    libdir = os.path.join(ROOT, "pynfam_amd")
+    subprocess.check_call(["amdflang", "-c", inc, "-o", "pnfam_gpu.o"], cwd=tmp_path)
+    subprocess.check_call(["amdflang", prog, "pnfam_gpu.o", "-L" + libdir, "-lpnfam_gpu", "-Wl,-rpath," + libdir,
+                           "-o", "chk"], cwd=tmp_path)
+    out = subprocess.run([str(tmp_path / "chk")], capture_output=True, text=True, check=True).stdout
+    assert "version: pnfam_gpu" in out and "gfx950" in out
+    assert "create rc: -1" in out
+    assert "nb must be positive" in out

@@ -1,0 +1,98 @@
+"""The pnfam_main.x drop-in: namelist input, .dat output in the format pynfam's
+parser reads (pynfam/outputs/pnfam_parser.py:33-132), and one archived point
+end to end on the GPU."""
+import json
+import os
+import shutil
+
+import numpy as np
+import pandas as pd
+import pytest
+
+from pynfam_amd import namelist
+from pynfam_amd import pnfam_main
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def test_namelist_reads_pynfam_file():
+    nml = namelist.read(os.path.join(GOLDEN, "S40", "GT-K0_000000.in"))
+    with open(os.path.join(GOLDEN, "S40", "GT-K0.json")) as fh:
+        ref = json.load(fh)[0]["namelist"]
+    assert nml["general"]["fam_output_filename"] == "GT-K0"
+    assert nml["general"]["real_eqrpa"] == ref["real_eqrpa"]
+    assert nml["general"]["imag_eqrpa"] == ref["imag_eqrpa"]
+    assert nml["ext_field"]["operator_name"] == "GT" and nml["ext_field"]["operator_k"] == 0
+    assert nml["ext_field"]["two_body_current_lecs"] == [-3.1962, 3.1962, 0]
+    assert nml["interaction"]["interaction_name"] == "SKOP"
+    assert nml["interaction"]["vpair_t0"] == -346.352
+    # "vpair_t1 = ," keeps the default (unset)
+    assert nml["interaction"]["vpair_t1"] == -namelist.HUGE
+    assert nml["solver"]["max_iter"] == 300 and nml["solver"]["convergence_epsilon"] == 1e-7
+    kw = pnfam_main.interaction_overrides(nml["interaction"])
+    assert kw["vpair_t0"] == -346.352 and kw["override_cs0"] == 128.279
+    assert "vpair_t1" not in kw and "override_ct" not in kw
+
+
+def test_namelist_defaults_and_errors(tmp_path):
+    p = tmp_path / "a.in"
+    p.write_text("&general\n real_eqrpa = 1.5d0, imag_eqrpa = 0.25 /\n&solver max_iter = 7 /\n")
+    nml = namelist.read(p)
+    assert nml["general"]["real_eqrpa"] == 1.5 and nml["general"]["imag_eqrpa"] == 0.25
+    assert nml["solver"]["max_iter"] == 7 and nml["solver"]["broyden_history_size"] == 50
+    assert nml["interaction"]["interaction_name"] == "NONE" and nml["ext_field"]["operator_name"] == "F"
+    p.write_text("&solver no_such_entry = 1 /\n")
+    with pytest.raises(namelist.NamelistError):
+        namelist.read(p)
+
+
+def _parse_like_pynfam(path):
+    """The parts of pnfam_parser.py the caller relies on."""
+    with open(path) as fh:
+        lines = fh.read().splitlines()
+    assert any("Total CPU time" in l for l in lines)
+    conv = any("iteration converged" in l for l in lines)
+    version = [l for l in lines if "Version:" in l][0].split()[-1]
+    df = pd.read_csv(path, sep=r"\s+", header=0, comment="#", index_col=0)
+    df = df.drop(df.index[0])
+    return conv, version, complex(df["Real"].values[0], df["Imag"].values[0])
+
+
+def test_dat_format_parses(tmp_path):
+    log = pnfam_main.Log(str(tmp_path / "x.dat"), False)
+    log.line(f"Version: {pnfam_main.VERSION}")
+    pnfam_main.write_result(log, complex(1.25, -0.5), complex(0.1234567890123456789, -2e-4), 15, 6.4e-8, 300, 0.6)
+    log.close()
+    conv, version, s = _parse_like_pynfam(tmp_path / "x.dat")
+    assert conv and version
+    assert s == complex(0.1234567890123456789, -2e-4)
+
+
+def test_errors_end_in_log_with_status_zero(tmp_path):
+    p = tmp_path / "bad.in"
+    p.write_text("&general fam_output_filename = 'bad' /\n&ext_field operator_name = 'RS2' /\n")
+    assert pnfam_main.run(str(p), workdir=str(tmp_path)) == 0
+    txt = (tmp_path / "bad.dat").read_text()
+    assert "ERROR" in txt and "Strength" not in txt
+    assert pnfam_main.main(["a", "b"]) == 0
+
+
+@pytest.mark.gpu
+def test_dropin_point_matches_archived_run(tmp_path):
+    """S40 GT- K=0, archived point 0: pnfam_main in a fresh directory with the
+    HFB files copied in, as pynfam's fortProcess does."""
+    for f in ("hfbtho_output.hel", "hfbtho_NAMELIST.dat", "GT-K0_000000.in"):
+        shutil.copy(os.path.join(GOLDEN, "S40", f), tmp_path / f)
+    cwd = os.getcwd()
+    os.chdir(tmp_path)
+    try:
+        assert pnfam_main.main(["GT-K0_000000.in"]) == 0
+    finally:
+        os.chdir(cwd)
+    conv, _, s = _parse_like_pynfam(tmp_path / "GT-K0.dat")
+    with open(os.path.join(GOLDEN, "S40", "GT-K0.json")) as fh:
+        ref = json.load(fh)[0]
+    assert conv
+    assert abs(s - complex(*ref["strength"])) < 1e-7 * abs(complex(*ref["strength"]))
+    txt = (tmp_path / "GT-K0.dat").read_text()
+    assert f"converged after {ref['iterations']:6d} steps" in txt
