@@ -14,6 +14,7 @@ workload on this host.
 """
 import argparse
 import json
+from concurrent.futures import ThreadPoolExecutor
 import os
 import sys
 import time
@@ -96,10 +97,18 @@ def measure(engines, w, precision, args):
     """Warm-up and timed steps of one precision path; per-kernel HIP-event totals."""
     import numpy as np
 
+    pool = ThreadPoolExecutor(max_workers=len(engines)) if args.concurrent else None
+
     def step():
+        # one host thread per operator-K context: each has its own stream, and the
+        # two batches share the GPU (ctypes releases the GIL during the solve)
+        if pool is not None:
+            futs = [pool.submit(e.solve, w, max_iter=args.max_iter, precision=precision) for e in engines]
+            rs = [f.result() for f in futs]
+        else:
+            rs = [e.solve(w, max_iter=args.max_iter, precision=precision) for e in engines]
         conv, its, S = 0, [], []
-        for e in engines:
-            r = e.solve(w, max_iter=args.max_iter, precision=precision)
+        for r in rs:
             ok = r["iterations"] >= 0
             conv += int(ok.sum())
             its.extend(r["iterations"][ok].tolist())
@@ -117,6 +126,8 @@ def measure(engines, w, precision, args):
         conv_total += c
         its_all.extend(its)
     dt = time.perf_counter() - t0
+    if pool is not None:
+        pool.shutdown()
     prof = {}
     for e in engines:
         for k, v in e.profile_read().items():
@@ -146,7 +157,8 @@ def rate_error(emax, npts, s_ref, s_mix):
     rmx, _, _ = gt_rates_from_half(zd, a, emax, 0.0, 2 * npts, {0: s_mix[:npts], 1: s_mix[npts:]})
     return {"total": abs(rmx["total"] - r64["total"]) / abs(r64["total"]),
             "K0": abs(rmx[0] - r64[0]) / abs(r64[0]), "K1": abs(rmx[1] - r64[1]) / abs(r64[1]),
-            "rate_fp64_per_s": r64["total"], "eqrpa_max_MeV": emax}
+            "rate_fp64_per_s": r64["total"], "eqrpa_max_MeV": emax,
+            "quadrature_cancellation": r64["cancellation"]}
 
 
 def main():
@@ -161,6 +173,8 @@ def main():
     ap.add_argument("--max-iter", type=int, default=300)
     ap.add_argument("--cpu-iters", type=int, default=10, help="oracle iterations in the CPU sample (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--concurrent", type=int, default=1,
+                    help="solve the K=0 and K=1 batches from two host threads at once (1) or in turn (0)")
     ap.add_argument("--precision", choices=["fp64", "mixed", "mixed_refine", "both"], default="both",
                     help="both: FP64 then mixed; the JSON value is the last one")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,

@@ -70,7 +70,11 @@ typedef struct {
  *     meets the tolerance, one full FP64 evaluation replaces the accumulated H
  *     and the point iterates on until it meets the tolerance again.
  *   MIXED_NOREFINE: as MIXED without the final FP64 evaluation; S then carries
- *     the low-precision error of the first increments (about 1e-5 relative). */
+ *     the low-precision error of the first increments (about 1e-6 relative).
+ *   In both mixed modes an increment whose size, judged by the last residual,
+ *   is below 1e-3 of the largest amplitude (environment PNFAM_GPU_MIX_TAU)
+ *   goes through single BF16 products instead of the three split products:
+ *   its rounding error is scaled down by the increment. */
 #define PNFAM_GPU_FP64 0
 #define PNFAM_GPU_MIXED 1
 #define PNFAM_GPU_MIXED_NOREFINE 2

@@ -16,7 +16,7 @@ FLAGS = ["-O3", "-std=c++17", "-fPIC", "-shared", "--offload-arch=gfx950",
 def build(verbose=False):
     src = os.path.join(HERE, "pnfam_gpu.hip")
     tmp = OUT + ".tmp"
-    cmd = [HIPCC] + FLAGS + ["-I", os.path.join(PKG, "..", "include"), "-o", tmp, src]
+    cmd = [HIPCC] + FLAGS + ["-I", os.path.join(PKG, "..", "include"), "-o", tmp, src, "-lrocblas"]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.check_call(cmd)

@@ -70,9 +70,34 @@ __global__ void k_mix_tables(TabArgs T) {
       v = (float)(T.psi[x][(size_t)a * T.nghl + r] * T.psi[y][(size_t)b * T.nghl + r]);
     __bf16 h, l;
     split_bf16(v, h, l);
-    const size_t o = (size_t)c * n + e;
+    // density layout [c][r][k]; projection layout [k][c][r] (K = (c, r) contiguous per row k)
+    const size_t o = T.dens ? (size_t)c * n + e : ((size_t)k * T.nc + c) * T.RP + r;
     T.hi[o] = h;
     T.lo[o] = l;
+  }
+}
+
+// FP32 fill (output planes of classes without elements)
+__global__ void k_fill_f32(float *__restrict__ p, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    p[i] = 0.0f;
+}
+
+// Projection results O[k][n] (FP32, n fastest) into the FP64 block matrices:
+// column n = (slot * nsec + sec) * 2 + ri, element elem[k] of quadrant q(sec).
+__global__ void k_mix_scatter(const float *__restrict__ O, long ldO, const int *__restrict__ elem, int Kp, int nS,
+                              int nsec, int q0, int q1, const int *__restrict__ act, double *__restrict__ hsp,
+                              long nxy) {
+  const int s = blockIdx.y * 64 + (threadIdx.x & 63);
+  if (s >= nS) return;
+  const int ri = s & 1;
+  const int sec = nsec == 2 ? (s >> 1) & 1 : 0;
+  const int slot = s / (2 * nsec);
+  const int q = sec ? q1 : q0;
+  double *out = hsp + ((size_t)act[slot] * 8 + 2 * q + ri) * nxy;
+  for (int k = blockIdx.x * 4 + (threadIdx.x >> 6); k < Kp; k += gridDim.x * 4) {
+    const int e = elem[k];
+    if (e >= 0) out[e] = (double)O[(size_t)k * ldO + s];
   }
 }
 
@@ -107,7 +132,7 @@ __global__ void k_mix_rho(CvtRhoArgs A) {
   }
 }
 
-// F[c][cls][n][r] (r fastest) from the FP64 fields [w][sec][208][r], times the
+// F[cls][n][c][r] (r fastest) from the FP64 fields [w][sec][208][r], times the
 // factor 2 of the projection (pnfam_hamiltonian_blas.f90:1106-1160, 1240-1262).
 // pair != 0: the four pairing fields of one sector, F[cls][n][r] with
 // n = slot * 2 + ri.  Otherwise the 18 mean-field pairs for both sectors.
@@ -153,7 +178,7 @@ __global__ void k_mix_fields(CvtFldArgs A) {
           const float v = live ? (float)(2.0 * f[(size_t)(2 * idx + ri) * A.nghl + r]) : 0.0f;
           __bf16 h, l;
           split_bf16(v, h, l);
-          const size_t o = (size_t)(c * 4 + cls) * plane + (size_t)((slot * 2 + sec) * 2 + ri) * A.RP + r;
+          const size_t o = (((size_t)cls * A.ncap + (slot * 2 + sec) * 2 + ri) * NPROJ_C + c) * A.RP + r;
           A.hi[o] = h;
           A.lo[o] = l;
         }

@@ -20,6 +20,8 @@
 #include <string>
 #include <vector>
 
+#include <rocblas/rocblas.h>
+
 #include "../../include/pnfam_gpu.h"
 #include "plan.hpp"
 
@@ -152,8 +154,8 @@ __global__ void __launch_bounds__(64) k_triprod_mfma(const Group *__restrict__ g
                                                      const double *__restrict__ m2, const double *__restrict__ m3,
                                                      const double *__restrict__ bbase, long bstride,
                                                      double *__restrict__ obase, long ostride,
-                                                     const int *__restrict__ act, long nxy) {
-  __shared__ double sT[16][16 * TP_MAXCT + 1];
+                                                     const int *__restrict__ act, long nxy, int pitch) {
+  extern __shared__ double sTdyn[];   // [16][pitch], pitch = 16 * (column tiles) + 1
   const Group g = groups[blockIdx.x];
   const int strip = blockIdx.z;
   const int w = act ? act[blockIdx.y] : (int)blockIdx.y;
@@ -197,14 +199,14 @@ __global__ void __launch_bounds__(64) k_triprod_mfma(const Group *__restrict__ g
     for (int c = 0; c < TP_MAXCT; ++c) {
       if (c >= nct) break;
 #pragma unroll
-      for (int v = 0; v < 4; ++v) sT[4 * v + mk][c * 16 + mi] = tm.s * T[c][v];
+      for (int v = 0; v < 4; ++v) sTdyn[(4 * v + mk) * pitch + c * 16 + mi] = tm.s * T[c][v];
     }
     __syncthreads();
     // phase 2: acc += T_strip op(C)
     for (int l0 = 0; l0 < ndc; l0 += 4) {
       const int l = l0 + mk;
       const bool lok = l < ndc;
-      const double tv = lok ? sT[mi][l] : 0.0;
+      const double tv = lok ? sTdyn[mi * pitch + l] : 0.0;
 #pragma unroll
       for (int c = 0; c < TP_MAXCT; ++c) {
         if (c >= nct) break;
@@ -461,24 +463,22 @@ __device__ inline void density_column(double *acc, const double (&Tr)[4][2], con
 
 // Local densities with FP64 MFMA (v_mfma_f64_16x16x4).  One wave per
 // (16 grid points, frequency, sector), looping over all block pairs.
-// T_x^s(r, b) = sum_a psi_x(r,a) rho(a,b) runs on the matrix cores (rows r,
-// k = a in steps of 4, 16 columns b per tile, re and im tiles), per-spin ranges
-// of a accumulated separately; the tile goes through LDS to a lane-per-(r, 4 b)
-// layout for the density products, whose sums over b stay in registers.
+// The tile is computed transposed, T^T(b, r) = sum_a rho(a, b) psi_x(r, a)
+// (rho as the A operand, psi as the B operand), so that lane r + 16 m,
+// register q holds T(r, b0 + 4 q + m): each lane owns one grid point and four
+// columns, and the density products run straight from the accumulators.
+// Per-spin ranges of a accumulate separately; the sums over the four column
+// subgroups are combined across lanes at the end.
 __global__ void __launch_bounds__(64) k_density_mfma(DensArgs A) {
-  __shared__ double sT[16][16][17];     // [(x, s, ri)][r][b]
   const int lane = threadIdx.x;
   const int r0 = blockIdx.x * 16;
   const int w = A.act[blockIdx.y];
   const int sec = blockIdx.z;
   const int G = A.nghl;
-  const int mi = lane & 15, mk = lane >> 4;     // MFMA operand lane coordinates
-  const int er = lane & 15, eg = lane >> 4;     // epilogue: row er, columns eg + 4q
+  const int mi = lane & 15, mk = lane >> 4;     // operand lane coordinates; epilogue: r = mi, b = 4 q + mk
   const int rA = r0 + mi;
   const bool rA_ok = rA < G;
-  const int rE = r0 + er;
-  const bool rE_ok = rE < G;
-  const int rEc = rE_ok ? rE : 0;
+  const int rAc = rA_ok ? rA : 0;
   double acc[2 * NDENS];
 #pragma unroll
   for (int d = 0; d < 2 * NDENS; ++d) acc[d] = 0.0;
@@ -503,7 +503,7 @@ __global__ void __launch_bounds__(64) k_density_mfma(DensArgs A) {
         for (int a = lo; a < hi; a += 4) {
           const int ak = a + mk;
           const bool ok = ak < hi;
-          const size_t oa = (size_t)(ia + (ok ? ak : lo)) * G + (rA_ok ? rA : 0);
+          const size_t oa = (size_t)(ia + (ok ? ak : lo)) * G + rAc;
           double av[4];
 #pragma unroll
           for (int x = 0; x < 4; ++x) av[x] = (ok && rA_ok) ? tab[x][oa] : 0.0;
@@ -512,37 +512,26 @@ __global__ void __launch_bounds__(64) k_density_mfma(DensArgs A) {
           const double br = bok ? rho_re[ob] : 0.0, bi = bok ? rho_im[ob] : 0.0;
 #pragma unroll
           for (int x = 0; x < 4; ++x) {
-            C[x][s2][0] = mfma64(av[x], br, C[x][s2][0]);
-            C[x][s2][1] = mfma64(av[x], bi, C[x][s2][1]);
+            C[x][s2][0] = mfma64(br, av[x], C[x][s2][0]);
+            C[x][s2][1] = mfma64(bi, av[x], C[x][s2][1]);
           }
         }
       }
 #pragma unroll
-      for (int x = 0; x < 4; ++x)
-#pragma unroll
-        for (int s2 = 0; s2 < 2; ++s2)
-#pragma unroll
-          for (int ri = 0; ri < 2; ++ri)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) sT[(x * 2 + s2) * 2 + ri][4 * v + mk][mi] = C[x][s2][ri][v];
-      __syncthreads();
       for (int q = 0; q < 4; ++q) {
-        const int bl = eg + 4 * q;
-        if (b0 + bl >= dy) break;
-        const int b = ib0 + b0 + bl;
-        double Tr[4][2], Ti[4][2];
+        const int bl = 4 * q + mk;
+        if (b0 + bl < dy) {
+          const int b = ib0 + b0 + bl;
+          double Tr[4][2], Ti[4][2];
 #pragma unroll
-        for (int x = 0; x < 4; ++x)
+          for (int x = 0; x < 4; ++x)
 #pragma unroll
-          for (int s2 = 0; s2 < 2; ++s2) {
-            Tr[x][s2] = sT[(x * 2 + s2) * 2 + 0][er][bl];
-            Ti[x][s2] = sT[(x * 2 + s2) * 2 + 1][er][bl];
-          }
-        const size_t oy = (size_t)b * G + rEc;
-        const double y[4] = {A.wf[oy], A.wfdr[oy], A.wfdp[oy], A.wfdz[oy]};
-        density_column(acc, Tr, Ti, y, A.ns[b]);
+            for (int s2 = 0; s2 < 2; ++s2) { Tr[x][s2] = C[x][s2][0][q]; Ti[x][s2] = C[x][s2][1][q]; }
+          const size_t oy = (size_t)b * G + rAc;
+          const double y[4] = {A.wf[oy], A.wfdr[oy], A.wfdp[oy], A.wfdz[oy]};
+          density_column(acc, Tr, Ti, y, A.ns[b]);
+        }
       }
-      __syncthreads();
     }
   }
   // pairing densities from kappa: x = 0 only
@@ -568,43 +557,37 @@ __global__ void __launch_bounds__(64) k_density_mfma(DensArgs A) {
           const bool bok = ok && bj < dy;
           const size_t ob = off + (size_t)(bok ? bj : 0) * dx + (ok ? ak : 0);
           const double br = bok ? kap_re[ob] : 0.0, bi = bok ? kap_im[ob] : 0.0;
-          C[s2][0] = mfma64(av, br, C[s2][0]);
-          C[s2][1] = mfma64(av, bi, C[s2][1]);
+          C[s2][0] = mfma64(br, av, C[s2][0]);
+          C[s2][1] = mfma64(bi, av, C[s2][1]);
         }
       }
 #pragma unroll
-      for (int s2 = 0; s2 < 2; ++s2)
-#pragma unroll
-        for (int ri = 0; ri < 2; ++ri)
-#pragma unroll
-          for (int v = 0; v < 4; ++v) sT[s2 * 2 + ri][4 * v + mk][mi] = C[s2][ri][v];
-      __syncthreads();
       for (int q = 0; q < 4; ++q) {
-        const int bl = eg + 4 * q;
-        if (b0 + bl >= dy) break;
-        const int b = ib0 + b0 + bl;
-        const int s = A.ns[b];
-        const int sd = s > 0 ? 0 : 1, so = 1 - sd;
-        const double wb = A.wf[(size_t)b * G + rEc];
-        double pr_ = 2.0 * sT[so * 2 + 0][er][bl] * wb, pi_ = 2.0 * sT[so * 2 + 1][er][bl] * wb;
-        acc[2 * D_RB] -= s * pr_; acc[2 * D_RB + 1] -= s * pi_;
-        acc[2 * D_SBZ] += pr_; acc[2 * D_SBZ + 1] += pi_;
-        pr_ = 2.0 * sT[sd * 2 + 0][er][bl] * wb; pi_ = 2.0 * sT[sd * 2 + 1][er][bl] * wb;
-        acc[2 * D_SBR] += s < 0 ? pr_ : -pr_; acc[2 * D_SBR + 1] += s < 0 ? pi_ : -pi_;
-        acc[2 * D_SBP] += pi_; acc[2 * D_SBP + 1] -= pr_;
+        const int bl = 4 * q + mk;
+        if (b0 + bl < dy) {
+          const int b = ib0 + b0 + bl;
+          const int s = A.ns[b];
+          const int sd = s > 0 ? 0 : 1, so = 1 - sd;
+          const double wb = A.wf[(size_t)b * G + rAc];
+          double pr_ = 2.0 * C[so][0][q] * wb, pi_ = 2.0 * C[so][1][q] * wb;
+          acc[2 * D_RB] -= s * pr_; acc[2 * D_RB + 1] -= s * pi_;
+          acc[2 * D_SBZ] += pr_; acc[2 * D_SBZ + 1] += pi_;
+          pr_ = 2.0 * C[sd][0][q] * wb; pi_ = 2.0 * C[sd][1][q] * wb;
+          acc[2 * D_SBR] += s < 0 ? pr_ : -pr_; acc[2 * D_SBR + 1] += s < 0 ? pi_ : -pi_;
+          acc[2 * D_SBP] += pi_; acc[2 * D_SBP + 1] -= pr_;
+        }
       }
-      __syncthreads();
     }
   }
-  // sum the four column subgroups (lanes er, er+16, er+32, er+48)
-  const double wc = rE_ok ? A.wdcori[rE] : 0.0;
-  double *out = A.dens + (((size_t)w * 2 + sec) * 2 * NDENS) * G + rEc;
+  // sum the four column subgroups (lanes r, r+16, r+32, r+48)
+  const double wc = rA_ok ? A.wdcori[rA] : 0.0;
+  double *out = A.dens + (((size_t)w * 2 + sec) * 2 * NDENS) * G + rAc;
 #pragma unroll
   for (int d = 0; d < 2 * NDENS; ++d) {
     double v = acc[d];
     v += __shfl_xor(v, 16, 64);
     v += __shfl_xor(v, 32, 64);
-    if (eg == 0 && rE_ok) out[(size_t)d * G] = v * wc;
+    if (mk == 0 && rA_ok) out[(size_t)d * G] = v * wc;
   }
 }
 
@@ -1028,6 +1011,7 @@ struct BroArgs {
   double *gram;           // [w][M][M]
   double *coef;           // [w][M] gamma
   double *si;             // [w]
+  double *vmax;           // [w] max |vin|
   double *scal;           // [w][4]: normi, ...
   long N;
   int M, nwg, iter, ipos, inext, iter_used;
@@ -1069,11 +1053,12 @@ __global__ void __launch_bounds__(BRO_WG) k_bro_resid(BroArgs B) {
   const bool hist = B.iter >= 2 && B.M > 0;
   double *dfp = B.df + ((size_t)w * B.M + (hist ? B.ipos : 0)) * N;
   double *dvp = B.dv + ((size_t)w * B.M + (hist ? B.ipos : 0)) * N;
-  double mx = 0.0, nr = 0.0;
+  double mx = 0.0, nr = 0.0, vm = 0.0;
   for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < N; e += (long)gridDim.x * blockDim.x) {
     const double d = vo[e] - vi[e];
     vo[e] = d;
     mx = fmax(mx, fabs(d));
+    vm = fmax(vm, fabs(vi[e]));
     if (hist) {
       const double f = d - dfp[e];
       dfp[e] = f;
@@ -1083,23 +1068,27 @@ __global__ void __launch_bounds__(BRO_WG) k_bro_resid(BroArgs B) {
   }
   const double m = wg_max(mx, red);
   const double s = wg_sum(nr, red);
+  const double v = wg_max(vm, red);
   if (threadIdx.x == 0) {
     double *pp = B.part + ((size_t)w * B.nwg + blockIdx.x) * (2 * BRO_MAXM + 2);
     pp[0] = m;
     pp[1] = s;
+    pp[2] = v;   // read by k_bro_norm before k_bro_dots reuses the slot
   }
 }
 
 __global__ void k_bro_norm(BroArgs B) {
   const int w = B.act[blockIdx.x];
   if (threadIdx.x != 0) return;
-  double m = 0.0, s = 0.0;
+  double m = 0.0, s = 0.0, v = 0.0;
   for (int k = 0; k < B.nwg; ++k) {
     const double *pp = B.part + ((size_t)w * B.nwg + k) * (2 * BRO_MAXM + 2);
     m = fmax(m, pp[0]);
     s += pp[1];
+    v = fmax(v, pp[2]);
   }
   B.si[w] = m;
+  B.vmax[w] = v;
   B.scal[(size_t)w * 4 + 0] = (B.iter >= 2 && B.M > 0) ? 1.0 / sqrt(s) : 1.0;
 }
 
@@ -1269,6 +1258,7 @@ struct pnfam_gpu_ctx {
   int ngroups_q2s[2] = {0, 0}, ngroups_s2q[2] = {0, 0};
   size_t lds_q2s = 0, lds_s2q = 0;
   int strips = 1;   // 16-row strips of the largest output block
+  int tp_pitch = 16 * TP_MAXCT + 1;   // LDS row pitch of the triprod strip (16 x widest block + 1)
   DevPtr d_pairs;   // rho0, kap0, rho1, kap1 pair lists
   int pair_off[4] = {0, 0, 0, 0}, pair_n[4] = {0, 0, 0, 0};
   DevPtr d_tiles, d_tiles2;
@@ -1277,7 +1267,7 @@ struct pnfam_gpu_ctx {
   // per-frequency workspace
   int cap = 0;
   int hist_cap = -1;
-  DevPtr d_vin, d_vout, d_rsp, d_hsp, d_hqp, d_dens, d_flds, d_part, d_cpart, d_S, d_si, d_scal;
+  DevPtr d_vin, d_vout, d_rsp, d_hsp, d_hqp, d_dens, d_flds, d_part, d_cpart, d_S, d_si, d_vmax, d_scal;
   DevPtr d_df, d_dv, d_gram, d_coef, d_wre, d_wim, d_act;
   // mixed path: last vin fed to the hamiltonian, its increment, accumulated FP64 H20/H02
   DevPtr d_act64, d_actlo, d_vprev, d_dvin, d_hacc;
@@ -1294,6 +1284,12 @@ struct pnfam_gpu_ctx {
   MixSet mix[3];                 // 0: mean field (f); 1: kappa+/Delta+ (sector 0); 2: kappa-/Delta- (sector 1)
   bool mix_ready = false;
   int RP = 0, mix_cap = 0;
+  // contractions: rocBLAS bf16 GEMMs (default) or the hand-written k_mix_gemm
+  // (PNFAM_GPU_MIX_GEMM=own)
+  rocblas_handle blas = nullptr;
+  bool own_gemm = false;
+  DevPtr d_Oproj;   // projection results [Kp][ncol] of the largest set (FP32)
+  DevPtr d_Psplit;  // split-K partial planes (FP32)
   int unit_precision = PNFAM_GPU_FP64;   // grid stage of the per-unit entry points
   int bro_nwg = 0;
   double flops_q2s = 0, flops_s2q = 0;
@@ -1306,6 +1302,7 @@ struct pnfam_gpu_ctx {
   std::vector<hipEvent_t> event_pool;
   double flops_unit[32] = {0};   // algorithmic FLOP per frequency, per kernel id
   ~pnfam_gpu_ctx() {
+    if (blas) (void)rocblas_destroy_handle(blas);
     for (auto &p : pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
     for (auto e : event_pool) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
@@ -1389,6 +1386,7 @@ int ensure_workspace(pnfam_gpu_ctx *h, int npts, int M) {
   if ((e = h->d_cpart.alloc(n * 64 * 2 * 8))) return e;
   if ((e = h->d_S.alloc(n * 2 * 8))) return e;
   if ((e = h->d_si.alloc(n * 8))) return e;
+  if ((e = h->d_vmax.alloc(n * 8))) return e;
   if ((e = h->d_scal.alloc(n * 4 * 8))) return e;
   if ((e = h->d_df.alloc(n * Mh * 4 * nxy * 8))) return e;
   if ((e = h->d_dv.alloc(n * Mh * 4 * nxy * 8))) return e;
@@ -1421,10 +1419,11 @@ int stage_qp2sp(pnfam_gpu_ctx *h, const double *vin, int na, const int *act) {
   for (int ri = 0; ri < 2; ++ri) {
     if (!h->ngroups_q2s[ri]) continue;
     if (!h->basic_kernels) {
-      hipLaunchKernelGGL(k_triprod_mfma, dim3(h->ngroups_q2s[ri], na, h->strips), dim3(64), 0, h->stream,
+      hipLaunchKernelGGL(k_triprod_mfma, dim3(h->ngroups_q2s[ri], na, h->strips), dim3(64),
+                         16 * h->tp_pitch * sizeof(double), h->stream,
                          h->d_groups_q2s[ri].as<Group>(), h->d_terms_q2s[ri].as<Term>(), h->d_U[0].as<double>(),
                          h->d_U[1].as<double>(), h->d_U[2].as<double>(), h->d_U[3].as<double>(), vin, 4 * nxy,
-                         h->d_rsp.as<double>(), 8 * nxy, act, nxy);
+                         h->d_rsp.as<double>(), 8 * nxy, act, nxy, h->tp_pitch);
       continue;
     }
     hipLaunchKernelGGL(k_triprod, dim3(h->ngroups_q2s[ri], na), dim3(256), h->lds_q2s, h->stream,
@@ -1561,6 +1560,13 @@ int mix_prepare(pnfam_gpu_ctx *h) {
   if (!same_pairs(h->rspmap[1], h->hspmap[1]) || !same_pairs(h->rspmap[2], h->hspmap[2]))
     return fail(PNFAM_GPU_EINVAL, "mixed path: kappa structure differs from Delta");
   h->RP = (h->nghl + MIX_RALIGN - 1) / MIX_RALIGN * MIX_RALIGN;
+  if (const char *ev = std::getenv("PNFAM_GPU_MIX_GEMM")) h->own_gemm = std::strcmp(ev, "own") == 0;
+  if (!h->own_gemm && !h->blas) {
+    if (rocblas_create_handle(&h->blas) != rocblas_status_success)
+      return fail(PNFAM_GPU_EDEVICE, "rocblas_create_handle failed");
+    if (rocblas_set_stream(h->blas, h->stream) != rocblas_status_success)
+      return fail(PNFAM_GPU_EDEVICE, "rocblas_set_stream failed");
+  }
   int e;
   if ((e = mix_build_set(h, h->mix[0], h->fmap, NPROJ_C, NDENS_C))) return e;
   if ((e = mix_build_set(h, h->mix[1], h->hspmap[1], 1, 1))) return e;
@@ -1580,6 +1586,8 @@ int mix_prepare(pnfam_gpu_ctx *h) {
   return 0;
 }
 
+constexpr int64_t MIX_KCHUNK = 2048;   // K chunk of the split-K contractions
+
 int mix_workspace(pnfam_gpu_ctx *h, int cap) {
   if (cap <= h->mix_cap) return 0;
   const size_t RP = h->RP;
@@ -1592,7 +1600,91 @@ int mix_workspace(pnfam_gpu_ctx *h, int cap) {
         (e = S.O.alloc(4 * (size_t)S.ntd * RP * ncol * 4)) || (e = S.Fh.alloc(nF * 2)) || (e = S.Fl.alloc(nF * 2)))
       return e;
   }
+  size_t kmax = 0;
+  for (int si = 0; si < 3; ++si) kmax = std::max(kmax, (size_t)h->mix[si].Kp);
+  if (int e = h->d_Oproj.alloc(kmax * 4 * (size_t)cap * 4)) return e;
+  // split-K planes of the pairing-set density contractions
+  size_t psz = 0;
+  for (int si = 1; si < 3; ++si) {
+    const auto &S = h->mix[si];
+    for (int c = 0; c < 4; ++c) {
+      const size_t kl = (size_t)(S.kbeg[c + 1] - S.kbeg[c]);
+      const size_t np = kl / MIX_KCHUNK + 1;
+      psz = std::max(psz, np * (size_t)S.ntd * RP * 2 * (size_t)cap);
+    }
+  }
+  if (int e = h->d_Psplit.alloc(std::max<size_t>(psz, 1) * 4)) return e;
   h->mix_cap = cap;
+  return 0;
+}
+
+// C[s][t] (t-major, ld = ldc) (+)= sum_k S[s][k] T[t][k]: column-major
+// C(s, t) = S^T(s, k) T(k, t) with both operands K-contiguous.
+int blas_nt(pnfam_gpu_ctx *h, int64_t ns, int64_t nt, int64_t k, const __bf16 *S, int64_t lds, const __bf16 *T,
+            int64_t ldt, float *C, int64_t ldc, bool accumulate) {
+  const float alpha = 1.0f, beta = accumulate ? 1.0f : 0.0f;
+  const rocblas_status st = rocblas_gemm_ex_64(h->blas, rocblas_operation_transpose, rocblas_operation_none, ns, nt, k,
+                                               &alpha, S, rocblas_datatype_bf16_r, lds, T, rocblas_datatype_bf16_r, ldt,
+                                               &beta, C, rocblas_datatype_f32_r, ldc, C, rocblas_datatype_f32_r, ldc,
+                                               rocblas_datatype_f32_r, rocblas_gemm_algo_standard, 0, 0);
+  if (st != rocblas_status_success) return fail(PNFAM_GPU_EDEVICE, "rocblas_gemm_ex failed: " + std::to_string(st));
+  return 0;
+}
+
+// As blas_nt for `batch` consecutive K chunks of length k: chunk j reads
+// S + j k, T + j k and writes its own output plane C + j cstride.
+int blas_nt_chunks(pnfam_gpu_ctx *h, int64_t ns, int64_t nt, int64_t k, int64_t batch, const __bf16 *S, int64_t lds,
+                   const __bf16 *T, int64_t ldt, float *C, int64_t ldc, int64_t cstride, bool accumulate) {
+  const float alpha = 1.0f, beta = accumulate ? 1.0f : 0.0f;
+  const rocblas_status st = rocblas_gemm_strided_batched_ex_64(
+      h->blas, rocblas_operation_transpose, rocblas_operation_none, ns, nt, k, &alpha, S, rocblas_datatype_bf16_r, lds,
+      k, T, rocblas_datatype_bf16_r, ldt, k, &beta, C, rocblas_datatype_f32_r, ldc, cstride, C, rocblas_datatype_f32_r,
+      ldc, cstride, batch, rocblas_datatype_f32_r, rocblas_gemm_algo_standard, 0, 0);
+  if (st != rocblas_status_success)
+    return fail(PNFAM_GPU_EDEVICE, "rocblas_gemm_strided_batched_ex failed: " + std::to_string(st));
+  return 0;
+}
+
+// The three split products hi*hi + hi*lo + lo*hi of one contraction.
+int blas_split(pnfam_gpu_ctx *h, int64_t ns, int64_t nt, int64_t k, const __bf16 *Sh, const __bf16 *Sl, int64_t lds,
+               const __bf16 *Th, const __bf16 *Tl, int64_t ldt, float *C, int64_t ldc, int nprod) {
+  if (int e = blas_nt(h, ns, nt, k, Sh, lds, Th, ldt, C, ldc, false)) return e;
+  if (nprod == 1) return 0;
+  if (int e = blas_nt(h, ns, nt, k, Sh, lds, Tl, ldt, C, ldc, true)) return e;
+  return blas_nt(h, ns, nt, k, Sl, lds, Th, ldt, C, ldc, true);
+}
+
+// Split-K form of blas_split for contractions with few output tiles (the pairing
+// densities: one grid-point row per table row): K chunks of MIX_KCHUNK go to
+// separate FP32 planes of P in one batched call each, the remainder to the next
+// plane, and k_sum_planes adds the planes into C.
+
+__global__ void k_sum_planes(const float *__restrict__ P, int nplanes, size_t plane, float *__restrict__ C) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < plane; i += (size_t)gridDim.x * blockDim.x) {
+    float v = 0.0f;
+    for (int j = 0; j < nplanes; ++j) v += P[(size_t)j * plane + i];
+    C[i] = v;
+  }
+}
+
+int blas_split_k(pnfam_gpu_ctx *h, int64_t ns, int64_t nt, int64_t k, const __bf16 *Sh, const __bf16 *Sl, int64_t lds,
+                 const __bf16 *Th, const __bf16 *Tl, int64_t ldt, float *C, int64_t ldc, int nprod, float *P,
+                 size_t pcap) {
+  const int64_t nfull = k / MIX_KCHUNK, rem = k - nfull * MIX_KCHUNK;
+  const int64_t plane = nt * ldc;
+  if ((size_t)(nfull + 1) * (size_t)plane > pcap) return fail(PNFAM_GPU_EINVAL, "split-K planes exceed their buffer");
+  const __bf16 *Ss[3] = {Sh, Sh, Sl}, *Ts[3] = {Th, Tl, Th};
+  for (int p = 0; p < nprod; ++p) {
+    if (nfull > 0)
+      if (int e = blas_nt_chunks(h, ns, nt, MIX_KCHUNK, nfull, Ss[p], lds, Ts[p], ldt, P, ldc, plane, p > 0)) return e;
+    if (rem > 0)
+      if (int e = blas_nt(h, ns, nt, rem, Ss[p] + nfull * MIX_KCHUNK, lds, Ts[p] + nfull * MIX_KCHUNK, ldt,
+                          P + nfull * plane, ldc, p > 0))
+        return e;
+  }
+  const int np = (int)(nfull + (rem > 0 ? 1 : 0));
+  hipLaunchKernelGGL(k_sum_planes, dim3(1024), dim3(256), 0, h->stream, P, np, (size_t)plane, C);
+  HIP_TRY(hipGetLastError());
   return 0;
 }
 
@@ -1605,7 +1697,7 @@ void mix_gemm(pnfam_gpu_ctx *h, GemmArgs G, const pnfam_gpu_ctx::MixSet &S, int 
 }
 
 // calc_hamiltonian for the active columns on split-BF16 MFMA: rsp -> hsp.
-int stage_hamiltonian_mixed(pnfam_gpu_ctx *h, int na, const int *act) {
+int stage_hamiltonian_mixed(pnfam_gpu_ctx *h, int na, const int *act, int nprod = 3) {
   if (int e = mix_workspace(h, h->cap)) return e;
   const size_t RP = h->RP;
   const int cap = h->mix_cap;
@@ -1622,7 +1714,31 @@ int stage_hamiltonian_mixed(pnfam_gpu_ctx *h, int na, const int *act) {
       hipLaunchKernelGGL(k_mix_rho, dim3((unsigned)std::min(1024, (S.Kp + 255) / 256), na), dim3(256), 0, h->stream, C);
     } }
   // 2. density contractions
-  { Prof pr(h, K_MIX_GEMM_D, na);
+  if (!h->own_gemm) {
+    Prof pr(h, K_MIX_GEMM_D, na);
+    for (int si = 0; si < 3; ++si) {
+      auto &S = h->mix[si];
+      const long ncol = (si == 0 ? 4 : 2) * (long)cap;
+      const int64_t nS = (si == 0 ? 4 : 2) * (int64_t)na, nT = (int64_t)S.ntd * RP;
+      for (int c = 0; c < 4; ++c) {
+        const int64_t k0 = S.kbeg[c], kl = S.kbeg[c + 1] - S.kbeg[c];
+        float *O = S.O.as<float>() + (size_t)c * nT * ncol;
+        if (kl == 0) {
+          hipLaunchKernelGGL(k_fill_f32, dim3(1024), dim3(256), 0, h->stream, O, (size_t)nT * ncol);
+          continue;
+        }
+        if (si != 0) {   // pairing sets: one table row per grid point, too few output tiles
+          if (int e = blas_split_k(h, nS, nT, kl, S.Bh.as<__bf16>() + k0, S.Bl.as<__bf16>() + k0, S.Kp,
+                                   S.TDh.as<__bf16>() + k0, S.TDl.as<__bf16>() + k0, S.Kp, O, ncol, nprod,
+                                   h->d_Psplit.as<float>(), h->d_Psplit.bytes / 4))
+            return e;
+        } else if (int e = blas_split(h, nS, nT, kl, S.Bh.as<__bf16>() + k0, S.Bl.as<__bf16>() + k0, S.Kp,
+                                      S.TDh.as<__bf16>() + k0, S.TDl.as<__bf16>() + k0, S.Kp, O, ncol, nprod))
+          return e;
+      }
+    }
+  } else {
+    Prof pr(h, K_MIX_GEMM_D, na);
     for (int si = 0; si < 3; ++si) {
       auto &S = h->mix[si];
       const long ncol = (si == 0 ? 4 : 2) * (long)cap;
@@ -1653,15 +1769,37 @@ int stage_hamiltonian_mixed(pnfam_gpu_ctx *h, int na, const int *act) {
       C.hi = S.Fh.as<__bf16>(); C.lo = S.Fl.as<__bf16>();
       hipLaunchKernelGGL(k_mix_fields, dim3((unsigned)((RP + 127) / 128), na), dim3(128), 0, h->stream, C);
     } }
-  // 6. projections straight into the FP64 block matrices
-  { Prof pr(h, K_MIX_GEMM_P, na);
+  // 6. projections: rocBLAS into O[k][n] and a scatter into the FP64 block
+  // matrices, or the hand-written kernel writing them directly
+  if (!h->own_gemm) {
+    Prof pr(h, K_MIX_GEMM_P, na);
+    for (int si = 0; si < 3; ++si) {
+      auto &S = h->mix[si];
+      const long ncol = (si == 0 ? 4 : 2) * (long)cap;
+      const int64_t nS = (si == 0 ? 4 : 2) * (int64_t)na, KK = (int64_t)S.ntp * RP;
+      float *O = h->d_Oproj.as<float>();
+      for (int c = 0; c < 4; ++c) {
+        const int64_t k0 = S.kbeg[c], kl = S.kbeg[c + 1] - S.kbeg[c];
+        if (kl == 0) continue;
+        const size_t fo = (size_t)c * ncol * KK;
+        if (int e = blas_split(h, nS, kl, KK, S.Fh.as<__bf16>() + fo, S.Fl.as<__bf16>() + fo, KK,
+                               S.TPh.as<__bf16>() + (size_t)k0 * KK, S.TPl.as<__bf16>() + (size_t)k0 * KK, KK,
+                               O + (size_t)k0 * ncol, ncol, nprod))
+          return e;
+      }
+      hipLaunchKernelGGL(k_mix_scatter, dim3((unsigned)std::min(2048, (S.Kp + 3) / 4), (unsigned)((nS + 63) / 64)),
+                         dim3(256), 0, h->stream, O, ncol, S.d_elem.as<int>(), S.Kp, (int)nS, si == 0 ? 2 : 1,
+                         si == 0 ? 0 : (si == 1 ? 1 : 2), 3, act, h->d_hsp.as<double>(), h->nxy);
+    }
+  } else {
+    Prof pr(h, K_MIX_GEMM_P, na);
     for (int si = 0; si < 3; ++si) {
       auto &S = h->mix[si];
       const long ncol = (si == 0 ? 4 : 2) * (long)cap;
       GemmArgs G{};
-      G.Th = S.TPh.as<__bf16>(); G.Tl = S.TPl.as<__bf16>(); G.ldT = RP; G.segT = (long)S.Kp * RP;
-      G.Sh = S.Fh.as<__bf16>(); G.Sl = S.Fl.as<__bf16>(); G.ldS = RP; G.segS = 4 * ncol * (long)RP;
-      G.nseg = S.ntp; G.nS = (si == 0 ? 4 : 2) * na; G.s_cls = ncol * (long)RP; G.o_cls = 0;
+      G.Th = S.TPh.as<__bf16>(); G.Tl = S.TPl.as<__bf16>(); G.ldT = (long)S.ntp * RP; G.segT = RP;
+      G.Sh = S.Fh.as<__bf16>(); G.Sl = S.Fl.as<__bf16>(); G.ldS = (long)S.ntp * RP; G.segS = RP;
+      G.nseg = S.ntp; G.nS = (si == 0 ? 4 : 2) * na; G.s_cls = ncol * (long)S.ntp * RP; G.o_cls = 0;
       G.jobs = S.d_jobs_proj.as<GemmJob>();
       G.mode = 1; G.hsp = h->d_hsp.as<double>(); G.elem = S.d_elem.as<int>(); G.act = act; G.nxy = h->nxy;
       G.nsec = si == 0 ? 2 : 1; G.q0 = si == 0 ? 0 : (si == 1 ? 1 : 2); G.q1 = 3;
@@ -1679,10 +1817,11 @@ int stage_sp2qp(pnfam_gpu_ctx *h, int na, const int *act) {
   for (int ri = 0; ri < 2; ++ri) {
     if (!h->ngroups_s2q[ri]) continue;
     if (!h->basic_kernels) {
-      hipLaunchKernelGGL(k_triprod_mfma, dim3(h->ngroups_s2q[ri], na, h->strips), dim3(64), 0, h->stream,
+      hipLaunchKernelGGL(k_triprod_mfma, dim3(h->ngroups_s2q[ri], na, h->strips), dim3(64),
+                         16 * h->tp_pitch * sizeof(double), h->stream,
                          h->d_groups_s2q[ri].as<Group>(), h->d_terms_s2q[ri].as<Term>(), h->d_U[0].as<double>(),
                          h->d_U[1].as<double>(), h->d_U[2].as<double>(), h->d_U[3].as<double>(),
-                         h->d_hsp.as<double>(), 8 * nxy, h->d_hqp.as<double>(), 4 * nxy, act, nxy);
+                         h->d_hsp.as<double>(), 8 * nxy, h->d_hqp.as<double>(), 4 * nxy, act, nxy, h->tp_pitch);
       continue;
     }
     hipLaunchKernelGGL(k_triprod, dim3(h->ngroups_s2q[ri], na), dim3(256), h->lds_s2q, h->stream,
@@ -1949,6 +2088,7 @@ int pnfam_gpu_set_operator(pnfam_gpu_ctx *h, int32_t beta_minus, const int32_t *
     for (int i = 0; i < nb; ++i) dmax = std::max(dmax, db[i]);
     if (dmax > 16 * TP_MAXCT) return fail(PNFAM_GPU_EINVAL, "block size above 128 is not supported");
     h->strips = (dmax + 15) / 16;
+    h->tp_pitch = 16 * h->strips + 1;   // strip columns never exceed the widest block
   }
   {
     Planner p; p.db = db;
@@ -2161,8 +2301,12 @@ int pnfam_gpu_solve_batch(pnfam_gpu_ctx *h, int32_t npts, const double *re_w, co
                        (size_t)npts * M * M, 0.0);
   const double *e1 = h->bminus ? h->d_Ep.as<double>() : h->d_En.as<double>();
   const double *e2 = h->bminus ? h->d_En.as<double>() : h->d_Ep.as<double>();
-  std::vector<double> si(npts, 1.0), S(2 * npts, 0.0);
+  std::vector<double> si(npts, 1.0), S(2 * npts, 0.0), vmax(npts, 1.0);
   std::vector<int> done(npts, 0), refined(npts, 0), act, act64, actlo;
+  // increments below tau of the amplitudes (judged by the last residual) go
+  // through single BF16 products: their rounding error is scaled by the increment
+  double tau = 1e-3;
+  if (const char *ev = std::getenv("PNFAM_GPU_MIX_TAU")) tau = std::atof(ev);
   for (int w = 0; w < npts; ++w) { iters[w] = -1; re_s[w] = 0.0; im_s[w] = 0.0; }
   if (mixed && !skip_res)
     hipLaunchKernelGGL(k_fill, grid1((size_t)npts * N, 256), dim3(256), 0, st, h->d_vprev.as<double>(),
@@ -2215,8 +2359,12 @@ int pnfam_gpu_solve_batch(pnfam_gpu_ctx *h, int32_t npts, const double *re_w, co
       { Prof pr(h, K_MIXACC, nlo);
         hipLaunchKernelGGL(k_delta, dim3(256, nlo), dim3(256), 0, st, h->d_vin.as<double>(), h->d_vprev.as<double>(),
                            h->d_dvin.as<double>(), N, dlo); }
+      double ratio = 0.0;
+      for (int w : actlo) ratio = std::max(ratio, si[w] / std::max(vmax[w], 1e-300));
+      const int nprod = (iter >= 3 && ratio < tau) ? 1 : 3;
       if (int e = stage_qp2sp(h, h->d_dvin.as<double>(), nlo, dlo)) return e;
-      if (int e = (h->basic_kernels ? stage_hamiltonian(h, nlo, dlo) : stage_hamiltonian_mixed(h, nlo, dlo))) return e;
+      if (int e = (h->basic_kernels ? stage_hamiltonian(h, nlo, dlo) : stage_hamiltonian_mixed(h, nlo, dlo, nprod)))
+        return e;
       if (int e = stage_sp2qp(h, nlo, dlo)) return e;
       { Prof pr(h, K_MIXACC, nlo);
         hipLaunchKernelGGL(k_hacc, dim3(256, nlo), dim3(256), 0, st, h->d_hacc.as<double>(), h->d_hqp.as<double>(),
@@ -2236,6 +2384,7 @@ int pnfam_gpu_solve_batch(pnfam_gpu_ctx *h, int32_t npts, const double *re_w, co
     B.vin = h->d_vin.as<double>(); B.vout = h->d_vout.as<double>(); B.df = h->d_df.as<double>();
     B.dv = h->d_dv.as<double>(); B.part = h->d_part.as<double>(); B.gram = h->d_gram.as<double>();
     B.coef = h->d_coef.as<double>(); B.si = h->d_si.as<double>(); B.scal = h->d_scal.as<double>();
+    B.vmax = h->d_vmax.as<double>();
     B.N = N; B.M = std::max(M, 0); B.nwg = nwg; B.iter = iter; B.alpha = o->alpha_mix; B.act = dact;
     int mode = 0;
     if (M < 0) mode = 2;
@@ -2262,6 +2411,7 @@ int pnfam_gpu_solve_batch(pnfam_gpu_ctx *h, int32_t npts, const double *re_w, co
       hipLaunchKernelGGL(k_contract_fin, dim3(na), dim3(64), 0, st, h->d_cpart.as<double>(), cnwg, h->d_S.as<double>(), dact); }
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(si.data(), h->d_si.p, npts * 8, hipMemcpyDeviceToHost, st));
+    if (mixed) HIP_TRY(hipMemcpyAsync(vmax.data(), h->d_vmax.p, npts * 8, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipMemcpyAsync(S.data(), h->d_S.p, 2 * npts * 8, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     if (h->prof) prof_collect(h);

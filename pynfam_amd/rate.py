@@ -165,13 +165,17 @@ def gt_rates(contour, strengths, f2):
     space f2 on the contour; returns {K: rate} and the total (K and -K counted
     through T_K^2)."""
     lam2 = (GA / GV) ** 2
-    rates = {}
+    rates, cond = {}, {}
     for K, s in strengths.items():
         t2 = 1.0 if K == 0 else 2.0
         c = lam2 * t2 * np.asarray(s) * f2
-        cint = np.sum(c * contour["dzdt"] * contour["weights"]) / (2j * np.pi)
+        terms = c * contour["dzdt"] * contour["weights"] / (2j * np.pi)
+        cint = np.sum(terms)
         rates[K] = float(np.log(2.0) / KAPPA * np.imag(-1j * np.pi * cint))
-    rates["total"] = float(sum(v for k, v in rates.items()))
+        # sum of |terms| over |rate integral|: how much the quadrature cancels
+        cond[K] = float(np.sum(np.abs(np.real(terms))) / max(abs(np.real(cint)), 1e-300))
+    rates["total"] = float(sum(rates[K] for K in strengths))
+    rates["cancellation"] = cond
     return rates
 
 

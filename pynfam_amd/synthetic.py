@@ -15,8 +15,8 @@ used for parity.
 * wavefunction tables: the same HO recurrences as real solutions (hfb.py);
 * U, V: seeded random block-orthogonal rotations with BCS u, v;
   E = sqrt((e - lambda)^2 + Delta^2), zeroed above a 60 MeV pairing window;
-* Fermi energies lambda_n = -5.5, lambda_p = -8.0 MeV (gaps 1.2, 1.0 MeV): a
-  beta-minus window EQRPA_max = Delta M_nH + lambda_n - lambda_p = 3.28 MeV
+* Fermi energies lambda_n = -2.0, lambda_p = -8.0 MeV (gaps 1.2, 1.0 MeV): a
+  beta-minus window EQRPA_max = Delta M_nH + lambda_n - lambda_p = 6.78 MeV
   above the lowest two-quasiparticle energies, as for a neutron-rich emitter.
 """
 from __future__ import annotations
@@ -28,7 +28,7 @@ import numpy as np
 from . import hfb
 
 SEED = 20261202
-LAMBDA_N, LAMBDA_P = -5.5, -8.0
+LAMBDA_N, LAMBDA_P = -2.0, -8.0
 
 
 def hfbtho_blocks(n00, beta, n00max=50):

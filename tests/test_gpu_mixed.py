@@ -34,13 +34,15 @@ def test_mixed_matches_reference_runs(nucleus, K, mode):
     eng.close()
 
 
+@pytest.mark.parametrize("gemm", ["rocblas", "own"])
 @pytest.mark.parametrize("K", [0, 1])
-def test_mixed_grid_stage_against_fp64(K):
+def test_mixed_grid_stage_against_fp64(K, gemm, monkeypatch):
     """One calc_hamiltonian through the split-BF16 pair-table contractions
-    against the FP64 grid stage on the same random densities: every output
-    array within 1e-4 of its largest element."""
+    (rocBLAS, or the hand-written k_mix_gemm) against the FP64 grid stage on the
+    same random densities: every output array within 1e-4 of its largest element."""
     from oracle.oracle import Oracle
     from pynfam_amd import engine as E
+    monkeypatch.setenv("PNFAM_GPU_MIX_GEMM", gemm)
     prob = golden_problem("S40", K)
     eng = E.GpuEngine(prob)
     orc = Oracle(prob)
@@ -62,7 +64,7 @@ def test_mixed_grid_stage_against_fp64(K):
     for k in range(8):
         scale = np.max(np.abs(ref[k]))
         err = np.max(np.abs(got[k] - ref[k])) / scale
-        print(f"K={K} arg {k}: max err / max {err:.2e}")
+        print(f"K={K} {gemm} arg {k}: max err / max {err:.2e}")
         assert err < 1e-4, (k, err)
     eng.close()
 
