@@ -115,9 +115,9 @@ __global__ void __launch_bounds__(256) k_triprod(const Group *__restrict__ group
   const int n = g.nrow * g.ncol;
   for (int t = 0; t < g.count; ++t) {
     const Term tm = terms[g.first + t];
-    const double *A = mats[tm.a_id] + tm.a_off;
+    const double *A = (tm.a_id == 0 ? m0 : tm.a_id == 1 ? m1 : tm.a_id == 2 ? m2 : m3) + tm.a_off;
     const double *Bb = B + (size_t)tm.b_slot * nxy + tm.b_off;
-    const double *C = mats[tm.c_id] + tm.c_off;
+    const double *C = (tm.c_id == 0 ? m0 : tm.c_id == 1 ? m1 : tm.c_id == 2 ? m2 : m3) + tm.c_off;
     const int nda = tm.nda, ndc = tm.ndc;
     for (int e = threadIdx.x; e < n; e += blockDim.x) {
       const int i = e % nda, j = e / nda;
@@ -149,13 +149,18 @@ constexpr int TP_MAXCT = 8;    // column tiles of 16 per strip (block sizes up t
 // strip of the output block, frequency): T_strip = s op(A)_strip op(B) is
 // accumulated over column tiles with v_mfma_f64_16x16x4, parked in LDS, and
 // multiplied by op(C); all terms of the group accumulate in registers.
+// Instantiated per number of 16-column tiles NCT (groups are sorted by it on
+// the host), so tile loops carry no branches and operand loads of a K step
+// issue together from clamped, always valid addresses.
+template <int NCT>
 __global__ void __launch_bounds__(64) k_triprod_mfma(const Group *__restrict__ groups, const Term *__restrict__ terms,
                                                      const double *__restrict__ m0, const double *__restrict__ m1,
                                                      const double *__restrict__ m2, const double *__restrict__ m3,
                                                      const double *__restrict__ bbase, long bstride,
                                                      double *__restrict__ obase, long ostride,
-                                                     const int *__restrict__ act, long nxy, int pitch) {
-  extern __shared__ double sTdyn[];   // [16][pitch], pitch = 16 * (column tiles) + 1
+                                                     const int *__restrict__ act, long nxy) {
+  constexpr int PITCH = 16 * NCT + 1;
+  __shared__ double sT[16 * PITCH];
   const Group g = groups[blockIdx.x];
   const int strip = blockIdx.z;
   const int w = act ? act[blockIdx.y] : (int)blockIdx.y;
@@ -164,63 +169,69 @@ __global__ void __launch_bounds__(64) k_triprod_mfma(const Group *__restrict__ g
   const int lane = threadIdx.x, mi = lane & 15, mk = lane >> 4;
   const double *B = bbase + (size_t)w * bstride;
   double *O = obase + (size_t)w * ostride + (size_t)g.out_slot * nxy + g.out_off;
-  const double *mats[4] = {m0, m1, m2, m3};
-  const int nct = (g.ncol + 15) / 16;
-  v4d acc[TP_MAXCT];
+  v4d acc[NCT];
 #pragma unroll
-  for (int c = 0; c < TP_MAXCT; ++c) acc[c] = v4d{0, 0, 0, 0};
+  for (int c = 0; c < NCT; ++c) acc[c] = v4d{0, 0, 0, 0};
   const int irow = i0 + mi;                      // operand row of op(A) for this lane
   const bool irow_ok = irow < g.nrow;
+  const int irc = irow_ok ? irow : 0;
   for (int t = 0; t < g.count; ++t) {
     const Term tm = terms[g.first + t];
-    const double *A = mats[tm.a_id] + tm.a_off;
+    const double *A = (tm.a_id == 0 ? m0 : tm.a_id == 1 ? m1 : tm.a_id == 2 ? m2 : m3) + tm.a_off;
     const double *Bb = B + (size_t)tm.b_slot * nxy + tm.b_off;
-    const double *C = mats[tm.c_id] + tm.c_off;
+    const double *C = (tm.c_id == 0 ? m0 : tm.c_id == 1 ? m1 : tm.c_id == 2 ? m2 : m3) + tm.c_off;
     const int nda = tm.nda, ndc = tm.ndc;
-    // phase 1: T_strip = s op(A)[i0:i0+16, :] op(B)
-    v4d T[TP_MAXCT];
+    // column indices of this lane per tile, clamped into the block
+    int jc[NCT];
+    double jm[NCT];
 #pragma unroll
-    for (int c = 0; c < TP_MAXCT; ++c) T[c] = v4d{0, 0, 0, 0};
+    for (int c = 0; c < NCT; ++c) {
+      const int j = c * 16 + mi;
+      jc[c] = j < ndc ? j : 0;
+      jm[c] = j < ndc ? 1.0 : 0.0;
+    }
+    // phase 1: T_strip = s op(A)[i0:i0+16, :] op(B).  The 0/1 factors (not
+    // selects) keep every load unconditional; clamped addresses hold finite data.
+    v4d T[NCT];
+#pragma unroll
+    for (int c = 0; c < NCT; ++c) T[c] = v4d{0, 0, 0, 0};
     for (int k0 = 0; k0 < nda; k0 += 4) {
       const int k = k0 + mk;
       const bool kok = k < nda;
-      const double av = (irow_ok && kok) ? (tm.ta ? A[k + (size_t)irow * nda] : A[irow + (size_t)k * nda]) : 0.0;
+      const int kc = kok ? k : 0;
+      const double km = kok ? 1.0 : 0.0;
+      const double av =
+          (tm.ta ? A[kc + (size_t)irc * nda] : A[irc + (size_t)kc * nda]) * (irow_ok ? km : 0.0);
+      double bv[NCT];
 #pragma unroll
-      for (int c = 0; c < TP_MAXCT; ++c) {
-        if (c >= nct) break;
-        const int j = c * 16 + mi;
-        const bool ok = kok && j < ndc;
-        const double bv = ok ? (tm.tb ? Bb[j + (size_t)k * tm.ldb] : Bb[k + (size_t)j * tm.ldb]) : 0.0;
-        T[c] = mfma64(av, bv, T[c]);
-      }
+      for (int c = 0; c < NCT; ++c)
+        bv[c] = (tm.tb ? Bb[jc[c] + (size_t)kc * tm.ldb] : Bb[kc + (size_t)jc[c] * tm.ldb]) * (km * jm[c]);
+#pragma unroll
+      for (int c = 0; c < NCT; ++c) T[c] = mfma64(av, bv[c], T[c]);
     }
     // park s*T in LDS: D[i][j] lives in lane j + 16 (i % 4), register i / 4
 #pragma unroll
-    for (int c = 0; c < TP_MAXCT; ++c) {
-      if (c >= nct) break;
+    for (int c = 0; c < NCT; ++c)
 #pragma unroll
-      for (int v = 0; v < 4; ++v) sTdyn[(4 * v + mk) * pitch + c * 16 + mi] = tm.s * T[c][v];
-    }
+      for (int v = 0; v < 4; ++v) sT[(4 * v + mk) * PITCH + c * 16 + mi] = tm.s * T[c][v];
     __syncthreads();
     // phase 2: acc += T_strip op(C)
     for (int l0 = 0; l0 < ndc; l0 += 4) {
       const int l = l0 + mk;
       const bool lok = l < ndc;
-      const double tv = lok ? sTdyn[mi * pitch + l] : 0.0;
+      const int lc = lok ? l : 0;
+      const double tv = sT[mi * PITCH + lc] * (lok ? 1.0 : 0.0);
+      double cv[NCT];
 #pragma unroll
-      for (int c = 0; c < TP_MAXCT; ++c) {
-        if (c >= nct) break;
-        const int j = c * 16 + mi;
-        const bool ok = lok && j < ndc;
-        const double cv = ok ? (tm.tc ? C[j + (size_t)l * ndc] : C[l + (size_t)j * ndc]) : 0.0;
-        acc[c] = mfma64(tv, cv, acc[c]);
-      }
+      for (int c = 0; c < NCT; ++c)
+        cv[c] = (tm.tc ? C[jc[c] + (size_t)lc * ndc] : C[lc + (size_t)jc[c] * ndc]) * jm[c];
+#pragma unroll
+      for (int c = 0; c < NCT; ++c) acc[c] = mfma64(tv, cv[c], acc[c]);
     }
     __syncthreads();
   }
 #pragma unroll
-  for (int c = 0; c < TP_MAXCT; ++c) {
-    if (c >= nct) break;
+  for (int c = 0; c < NCT; ++c) {
     const int j = c * 16 + mi;
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
@@ -463,22 +474,24 @@ __device__ inline void density_column(double *acc, const double (&Tr)[4][2], con
 
 // Local densities with FP64 MFMA (v_mfma_f64_16x16x4).  One wave per
 // (16 grid points, frequency, sector), looping over all block pairs.
-// The tile is computed transposed, T^T(b, r) = sum_a rho(a, b) psi_x(r, a)
-// (rho as the A operand, psi as the B operand), so that lane r + 16 m,
-// register q holds T(r, b0 + 4 q + m): each lane owns one grid point and four
-// columns, and the density products run straight from the accumulators.
-// Per-spin ranges of a accumulate separately; the sums over the four column
-// subgroups are combined across lanes at the end.
+// T_x^s(r, b) = sum_a psi_x(r,a) rho(a,b) runs on the matrix cores (rows r,
+// k = a in steps of 4, 16 columns b per tile, re and im tiles), per-spin ranges
+// of a accumulated separately; the tile goes through LDS to a lane-per-(r, 4 b)
+// layout for the density products, whose sums over b stay in registers.
 __global__ void __launch_bounds__(64) k_density_mfma(DensArgs A) {
+  __shared__ double sT[16][16][17];     // [(x, s, ri)][r][b]
   const int lane = threadIdx.x;
   const int r0 = blockIdx.x * 16;
   const int w = A.act[blockIdx.y];
   const int sec = blockIdx.z;
   const int G = A.nghl;
-  const int mi = lane & 15, mk = lane >> 4;     // operand lane coordinates; epilogue: r = mi, b = 4 q + mk
+  const int mi = lane & 15, mk = lane >> 4;     // MFMA operand lane coordinates
+  const int er = lane & 15, eg = lane >> 4;     // epilogue: row er, columns eg + 4q
   const int rA = r0 + mi;
   const bool rA_ok = rA < G;
-  const int rAc = rA_ok ? rA : 0;
+  const int rE = r0 + er;
+  const bool rE_ok = rE < G;
+  const int rEc = rE_ok ? rE : 0;
   double acc[2 * NDENS];
 #pragma unroll
   for (int d = 0; d < 2 * NDENS; ++d) acc[d] = 0.0;
@@ -503,7 +516,7 @@ __global__ void __launch_bounds__(64) k_density_mfma(DensArgs A) {
         for (int a = lo; a < hi; a += 4) {
           const int ak = a + mk;
           const bool ok = ak < hi;
-          const size_t oa = (size_t)(ia + (ok ? ak : lo)) * G + rAc;
+          const size_t oa = (size_t)(ia + (ok ? ak : lo)) * G + (rA_ok ? rA : 0);
           double av[4];
 #pragma unroll
           for (int x = 0; x < 4; ++x) av[x] = (ok && rA_ok) ? tab[x][oa] : 0.0;
@@ -512,26 +525,37 @@ __global__ void __launch_bounds__(64) k_density_mfma(DensArgs A) {
           const double br = bok ? rho_re[ob] : 0.0, bi = bok ? rho_im[ob] : 0.0;
 #pragma unroll
           for (int x = 0; x < 4; ++x) {
-            C[x][s2][0] = mfma64(br, av[x], C[x][s2][0]);
-            C[x][s2][1] = mfma64(bi, av[x], C[x][s2][1]);
+            C[x][s2][0] = mfma64(av[x], br, C[x][s2][0]);
+            C[x][s2][1] = mfma64(av[x], bi, C[x][s2][1]);
           }
         }
       }
 #pragma unroll
+      for (int x = 0; x < 4; ++x)
+#pragma unroll
+        for (int s2 = 0; s2 < 2; ++s2)
+#pragma unroll
+          for (int ri = 0; ri < 2; ++ri)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) sT[(x * 2 + s2) * 2 + ri][4 * v + mk][mi] = C[x][s2][ri][v];
+      __syncthreads();
       for (int q = 0; q < 4; ++q) {
-        const int bl = 4 * q + mk;
-        if (b0 + bl < dy) {
-          const int b = ib0 + b0 + bl;
-          double Tr[4][2], Ti[4][2];
+        const int bl = eg + 4 * q;
+        if (b0 + bl >= dy) break;
+        const int b = ib0 + b0 + bl;
+        double Tr[4][2], Ti[4][2];
 #pragma unroll
-          for (int x = 0; x < 4; ++x)
+        for (int x = 0; x < 4; ++x)
 #pragma unroll
-            for (int s2 = 0; s2 < 2; ++s2) { Tr[x][s2] = C[x][s2][0][q]; Ti[x][s2] = C[x][s2][1][q]; }
-          const size_t oy = (size_t)b * G + rAc;
-          const double y[4] = {A.wf[oy], A.wfdr[oy], A.wfdp[oy], A.wfdz[oy]};
-          density_column(acc, Tr, Ti, y, A.ns[b]);
-        }
+          for (int s2 = 0; s2 < 2; ++s2) {
+            Tr[x][s2] = sT[(x * 2 + s2) * 2 + 0][er][bl];
+            Ti[x][s2] = sT[(x * 2 + s2) * 2 + 1][er][bl];
+          }
+        const size_t oy = (size_t)b * G + rEc;
+        const double y[4] = {A.wf[oy], A.wfdr[oy], A.wfdp[oy], A.wfdz[oy]};
+        density_column(acc, Tr, Ti, y, A.ns[b]);
       }
+      __syncthreads();
     }
   }
   // pairing densities from kappa: x = 0 only
@@ -557,37 +581,43 @@ __global__ void __launch_bounds__(64) k_density_mfma(DensArgs A) {
           const bool bok = ok && bj < dy;
           const size_t ob = off + (size_t)(bok ? bj : 0) * dx + (ok ? ak : 0);
           const double br = bok ? kap_re[ob] : 0.0, bi = bok ? kap_im[ob] : 0.0;
-          C[s2][0] = mfma64(br, av, C[s2][0]);
-          C[s2][1] = mfma64(bi, av, C[s2][1]);
+          C[s2][0] = mfma64(av, br, C[s2][0]);
+          C[s2][1] = mfma64(av, bi, C[s2][1]);
         }
       }
 #pragma unroll
+      for (int s2 = 0; s2 < 2; ++s2)
+#pragma unroll
+        for (int ri = 0; ri < 2; ++ri)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) sT[s2 * 2 + ri][4 * v + mk][mi] = C[s2][ri][v];
+      __syncthreads();
       for (int q = 0; q < 4; ++q) {
-        const int bl = 4 * q + mk;
-        if (b0 + bl < dy) {
-          const int b = ib0 + b0 + bl;
-          const int s = A.ns[b];
-          const int sd = s > 0 ? 0 : 1, so = 1 - sd;
-          const double wb = A.wf[(size_t)b * G + rAc];
-          double pr_ = 2.0 * C[so][0][q] * wb, pi_ = 2.0 * C[so][1][q] * wb;
-          acc[2 * D_RB] -= s * pr_; acc[2 * D_RB + 1] -= s * pi_;
-          acc[2 * D_SBZ] += pr_; acc[2 * D_SBZ + 1] += pi_;
-          pr_ = 2.0 * C[sd][0][q] * wb; pi_ = 2.0 * C[sd][1][q] * wb;
-          acc[2 * D_SBR] += s < 0 ? pr_ : -pr_; acc[2 * D_SBR + 1] += s < 0 ? pi_ : -pi_;
-          acc[2 * D_SBP] += pi_; acc[2 * D_SBP + 1] -= pr_;
-        }
+        const int bl = eg + 4 * q;
+        if (b0 + bl >= dy) break;
+        const int b = ib0 + b0 + bl;
+        const int s = A.ns[b];
+        const int sd = s > 0 ? 0 : 1, so = 1 - sd;
+        const double wb = A.wf[(size_t)b * G + rEc];
+        double pr_ = 2.0 * sT[so * 2 + 0][er][bl] * wb, pi_ = 2.0 * sT[so * 2 + 1][er][bl] * wb;
+        acc[2 * D_RB] -= s * pr_; acc[2 * D_RB + 1] -= s * pi_;
+        acc[2 * D_SBZ] += pr_; acc[2 * D_SBZ + 1] += pi_;
+        pr_ = 2.0 * sT[sd * 2 + 0][er][bl] * wb; pi_ = 2.0 * sT[sd * 2 + 1][er][bl] * wb;
+        acc[2 * D_SBR] += s < 0 ? pr_ : -pr_; acc[2 * D_SBR + 1] += s < 0 ? pi_ : -pi_;
+        acc[2 * D_SBP] += pi_; acc[2 * D_SBP + 1] -= pr_;
       }
+      __syncthreads();
     }
   }
-  // sum the four column subgroups (lanes r, r+16, r+32, r+48)
-  const double wc = rA_ok ? A.wdcori[rA] : 0.0;
-  double *out = A.dens + (((size_t)w * 2 + sec) * 2 * NDENS) * G + rAc;
+  // sum the four column subgroups (lanes er, er+16, er+32, er+48)
+  const double wc = rE_ok ? A.wdcori[rE] : 0.0;
+  double *out = A.dens + (((size_t)w * 2 + sec) * 2 * NDENS) * G + rEc;
 #pragma unroll
   for (int d = 0; d < 2 * NDENS; ++d) {
     double v = acc[d];
     v += __shfl_xor(v, 16, 64);
     v += __shfl_xor(v, 32, 64);
-    if (mk == 0 && rA_ok) out[(size_t)d * G] = v * wc;
+    if (eg == 0 && rE_ok) out[(size_t)d * G] = v * wc;
   }
 }
 
@@ -1258,7 +1288,8 @@ struct pnfam_gpu_ctx {
   int ngroups_q2s[2] = {0, 0}, ngroups_s2q[2] = {0, 0};
   size_t lds_q2s = 0, lds_s2q = 0;
   int strips = 1;   // 16-row strips of the largest output block
-  int tp_pitch = 16 * TP_MAXCT + 1;   // LDS row pitch of the triprod strip (16 x widest block + 1)
+  // triprod groups sorted by column tiles: [stage q2s/s2q][ri][nct] -> first group, count, strips
+  int tp_first[2][2][TP_MAXCT + 1] = {}, tp_count[2][2][TP_MAXCT + 1] = {}, tp_strips[2][2][TP_MAXCT + 1] = {};
   DevPtr d_pairs;   // rho0, kap0, rho1, kap1 pair lists
   int pair_off[4] = {0, 0, 0, 0}, pair_n[4] = {0, 0, 0, 0};
   DevPtr d_tiles, d_tiles2;
@@ -1368,6 +1399,49 @@ std::vector<int4> pair_list(const BlockMap &m) {
   return v;
 }
 
+// Orders a stage's groups by their number of 16-column tiles and records, per
+// tile count, the first group, the count and the 16-row strips of the tallest.
+void sort_groups_by_tiles(pnfam_gpu_ctx *h, int stage, int ri, std::vector<Group> &groups) {
+  auto nct = [](const Group &g) { return (g.ncol + 15) / 16; };
+  std::stable_sort(groups.begin(), groups.end(), [&](const Group &a, const Group &b) { return nct(a) < nct(b); });
+  for (int c = 0; c <= TP_MAXCT; ++c) { h->tp_first[stage][ri][c] = 0; h->tp_count[stage][ri][c] = 0; h->tp_strips[stage][ri][c] = 0; }
+  for (size_t i = 0; i < groups.size(); ++i) {
+    const int c = nct(groups[i]);
+    if (h->tp_count[stage][ri][c] == 0) h->tp_first[stage][ri][c] = (int)i;
+    h->tp_count[stage][ri][c]++;
+    h->tp_strips[stage][ri][c] = std::max(h->tp_strips[stage][ri][c], (groups[i].nrow + 15) / 16);
+  }
+}
+
+// One launch per tile count present in the stage.
+int launch_triprod(pnfam_gpu_ctx *h, int stage, int ri, const Group *groups, const Term *terms, const double *bbase,
+                   long bstride, double *obase, long ostride, int na, const int *act) {
+  const double *m0 = h->d_U[0].as<double>(), *m1 = h->d_U[1].as<double>(), *m2 = h->d_U[2].as<double>(),
+               *m3 = h->d_U[3].as<double>();
+  for (int c = 1; c <= TP_MAXCT; ++c) {
+    const int n = h->tp_count[stage][ri][c];
+    if (!n) continue;
+    const dim3 grid(n, na, h->tp_strips[stage][ri][c]);
+    const Group *g = groups + h->tp_first[stage][ri][c];
+#define TP_LAUNCH(NC)                                                                                      \
+  hipLaunchKernelGGL(k_triprod_mfma<NC>, grid, dim3(64), 0, h->stream, g, terms, m0, m1, m2, m3, bbase, bstride, \
+                     obase, ostride, act, h->nxy)
+    switch (c) {
+      case 1: TP_LAUNCH(1); break;
+      case 2: TP_LAUNCH(2); break;
+      case 3: TP_LAUNCH(3); break;
+      case 4: TP_LAUNCH(4); break;
+      case 5: TP_LAUNCH(5); break;
+      case 6: TP_LAUNCH(6); break;
+      case 7: TP_LAUNCH(7); break;
+      default: TP_LAUNCH(8); break;
+    }
+#undef TP_LAUNCH
+  }
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
 int ensure_workspace(pnfam_gpu_ctx *h, int npts, int M) {
   if (npts <= h->cap && M == h->hist_cap) return 0;
   const size_t n = (size_t)std::max(npts, 1);
@@ -1419,11 +1493,9 @@ int stage_qp2sp(pnfam_gpu_ctx *h, const double *vin, int na, const int *act) {
   for (int ri = 0; ri < 2; ++ri) {
     if (!h->ngroups_q2s[ri]) continue;
     if (!h->basic_kernels) {
-      hipLaunchKernelGGL(k_triprod_mfma, dim3(h->ngroups_q2s[ri], na, h->strips), dim3(64),
-                         16 * h->tp_pitch * sizeof(double), h->stream,
-                         h->d_groups_q2s[ri].as<Group>(), h->d_terms_q2s[ri].as<Term>(), h->d_U[0].as<double>(),
-                         h->d_U[1].as<double>(), h->d_U[2].as<double>(), h->d_U[3].as<double>(), vin, 4 * nxy,
-                         h->d_rsp.as<double>(), 8 * nxy, act, nxy, h->tp_pitch);
+      if (int e = launch_triprod(h, 0, ri, h->d_groups_q2s[ri].as<Group>(), h->d_terms_q2s[ri].as<Term>(), vin, 4 * nxy,
+                                 h->d_rsp.as<double>(), 8 * nxy, na, act))
+        return e;
       continue;
     }
     hipLaunchKernelGGL(k_triprod, dim3(h->ngroups_q2s[ri], na), dim3(256), h->lds_q2s, h->stream,
@@ -1817,11 +1889,9 @@ int stage_sp2qp(pnfam_gpu_ctx *h, int na, const int *act) {
   for (int ri = 0; ri < 2; ++ri) {
     if (!h->ngroups_s2q[ri]) continue;
     if (!h->basic_kernels) {
-      hipLaunchKernelGGL(k_triprod_mfma, dim3(h->ngroups_s2q[ri], na, h->strips), dim3(64),
-                         16 * h->tp_pitch * sizeof(double), h->stream,
-                         h->d_groups_s2q[ri].as<Group>(), h->d_terms_s2q[ri].as<Term>(), h->d_U[0].as<double>(),
-                         h->d_U[1].as<double>(), h->d_U[2].as<double>(), h->d_U[3].as<double>(),
-                         h->d_hsp.as<double>(), 8 * nxy, h->d_hqp.as<double>(), 4 * nxy, act, nxy, h->tp_pitch);
+      if (int e = launch_triprod(h, 1, ri, h->d_groups_s2q[ri].as<Group>(), h->d_terms_s2q[ri].as<Term>(),
+                                 h->d_hsp.as<double>(), 8 * nxy, h->d_hqp.as<double>(), 4 * nxy, na, act))
+        return e;
       continue;
     }
     hipLaunchKernelGGL(k_triprod, dim3(h->ngroups_s2q[ri], na), dim3(256), h->lds_s2q, h->stream,
@@ -1972,6 +2042,7 @@ int pnfam_gpu_set_operator(pnfam_gpu_ctx *h, int32_t beta_minus, const int32_t *
     h->lds_q2s = std::max(h->lds_q2s, lds);
     if (ri == 0) h->flops_q2s = 0;
     for (auto &t : terms) h->flops_q2s += 2.0 * t.nda * t.nda * t.ndc + 2.0 * t.nda * t.ndc * t.ndc;
+    sort_groups_by_tiles(h, 0, ri, groups);
     int e;
     if ((e = upload(h->d_terms_q2s[ri], terms.data(), std::max<size_t>(1, terms.size()) * sizeof(Term), h->stream)) ||
         (e = upload(h->d_groups_q2s[ri], groups.data(), std::max<size_t>(1, groups.size()) * sizeof(Group), h->stream)))
@@ -2006,6 +2077,7 @@ int pnfam_gpu_set_operator(pnfam_gpu_ctx *h, int32_t beta_minus, const int32_t *
     h->lds_s2q = std::max(h->lds_s2q, lds);
     if (ri == 0) h->flops_s2q = 0;
     for (auto &t : terms) h->flops_s2q += 2.0 * t.nda * t.nda * t.ndc + 2.0 * t.nda * t.ndc * t.ndc;
+    sort_groups_by_tiles(h, 1, ri, groups);
     int e;
     if ((e = upload(h->d_terms_s2q[ri], terms.data(), std::max<size_t>(1, terms.size()) * sizeof(Term), h->stream)) ||
         (e = upload(h->d_groups_s2q[ri], groups.data(), std::max<size_t>(1, groups.size()) * sizeof(Group), h->stream)))
@@ -2088,7 +2160,6 @@ int pnfam_gpu_set_operator(pnfam_gpu_ctx *h, int32_t beta_minus, const int32_t *
     for (int i = 0; i < nb; ++i) dmax = std::max(dmax, db[i]);
     if (dmax > 16 * TP_MAXCT) return fail(PNFAM_GPU_EINVAL, "block size above 128 is not supported");
     h->strips = (dmax + 15) / 16;
-    h->tp_pitch = 16 * h->strips + 1;   // strip columns never exceed the widest block
   }
   {
     Planner p; p.db = db;
