@@ -858,13 +858,18 @@ constexpr int P2_MAXRT = 4;   // row tiles of 16 per wave
 
 // h_ab / Delta_ab with FP64 MFMA: D[a][b] = sum_{z,r} psi_z(r,a) B_z(r,b), with
 // B_z(r,b) = sum_z' mf(z,z',s_a,s_b)(r) psi_z'(r,b) built in registers from
-// fields staged in LDS per 16 grid points (kind 1: B = P_{s_a s_b}(r) psi_0(r,b)).
-// Rows: up to 64 states of spin s_a of block ix; columns: 16 states of block iy.
+// fields staged in LDS per 16 grid points (KIND 1: B = P_{s_a s_b}(r) psi_0(r,b)).
+// Rows: NRT x 16 states of spin s_a of block ix; columns: 16 states of block iy.
+// Instantiated per (NRT, KIND) with tiles sorted accordingly on the host, so the
+// row-tile and table loops carry no branches.  The fields are staged with the
+// grid point fastest across lanes (128-byte reads) and no integer division.
 // Grid (tile, frequency), tiles fastest.  Ordering frequencies fastest within
-// an XCD (so a tile's psi rows are fetched once per L2) cuts the HBM fetch 4x
+// an XCD (so a tile's psi rows are fetched once per L2) cut the HBM fetch 4x
 // but ran 1.4x slower: the kernel is not bound by HBM traffic.
+template <int NRT, int KIND>
 __global__ void __launch_bounds__(64) k_project_mfma(ProjArgs P, const Tile2 *__restrict__ tiles) {
-  __shared__ double sMF[16][2][5][5][2];   // [r][s_b][z][z'][ri]
+  constexpr int NZ = KIND == 0 ? 5 : 1;
+  __shared__ double sMF[16][2][25][2];     // [r][s_b][z * 5 + z'][ri]
   const Tile2 t = tiles[blockIdx.x];
   const int w = P.act[blockIdx.y];
   const int G = P.nghl;
@@ -873,30 +878,29 @@ __global__ void __launch_bounds__(64) k_project_mfma(ProjArgs P, const Tile2 *__
   const int ia = P.isstart[t.ix], ib0 = P.isstart[t.iy];
   const int dx = P.db[t.ix], dy = P.db[t.iy];
   const int nrows = t.a_hi - t.a_lo;
-  const int nrt = (nrows + 15) / 16;
-  const int nz = t.kind == 0 ? 5 : 1;
   const double *fl = P.flds + (((size_t)w * 2 + t.sec) * 2 * NFLD) * G;
   const int bj = t.b0 + mi;             // this lane's B column
   const bool bcol = bj < dy;
   const int sb = bcol ? (P.ns[ib0 + bj] > 0 ? 0 : 1) : 0;
-  v4d C[P2_MAXRT][2];
+  v4d C[NRT][2];
 #pragma unroll
-  for (int q = 0; q < P2_MAXRT; ++q) { C[q][0] = v4d{0, 0, 0, 0}; C[q][1] = v4d{0, 0, 0, 0}; }
+  for (int q = 0; q < NRT; ++q) { C[q][0] = v4d{0, 0, 0, 0}; C[q][1] = v4d{0, 0, 0, 0}; }
   for (int rc = 0; rc < G; rc += 16) {
-    // stage the fields of 16 grid points for this spin of a
-    if (t.kind == 0) {
+    // stage the fields of 16 grid points for this spin of a; lanes run along r
+    if (KIND == 0) {
       for (int e = lane; e < 16 * 2 * 25 * 2; e += 64) {
-        const int ri = e & 1, zz = (e >> 1) % 25, sbb = ((e >> 1) / 25) & 1, rr = (e >> 1) / 50;
+        const int rr = e & 15, q = e >> 4, ri = q & 1, jz = q >> 1;   // jz = s_b * 25 + z * 5 + z'
+        const int sbb = jz >= 25 ? 1 : 0, zz = jz - 25 * sbb;
         const int r = rc + rr;
-        const int idx = mf_index(zz / 5, zz % 5, t.sa, sbb);
-        sMF[rr][sbb][zz / 5][zz % 5][ri] = r < G ? fl[(size_t)(2 * idx + ri) * G + r] : 0.0;
+        const int idx = (zz * 2 + t.sa) * 2 + sbb;                      // mf_index(z, z', s_a, s_b)
+        sMF[rr][sbb][zz][ri] = r < G ? fl[(size_t)(2 * idx + ri) * G + r] : 0.0;
       }
     } else {
       for (int e = lane; e < 16 * 2 * 2; e += 64) {
-        const int ri = e & 1, sbb = (e >> 1) & 1, rr = e >> 2;
+        const int rr = e & 15, ri = (e >> 4) & 1, sbb = e >> 5;
         const int r = rc + rr;
         const int idx = 100 + (sbb == 0 ? (t.sa == 0 ? 0 : 1) : (t.sa == 0 ? 2 : 3));
-        sMF[rr][sbb][0][0][ri] = r < G ? fl[(size_t)(2 * idx + ri) * G + r] : 0.0;
+        sMF[rr][sbb][0][ri] = r < G ? fl[(size_t)(2 * idx + ri) * G + r] : 0.0;
       }
     }
     __syncthreads();
@@ -906,32 +910,30 @@ __global__ void __launch_bounds__(64) k_project_mfma(ProjArgs P, const Tile2 *__
       const int r = rc + rr;
       const bool rok = r < G;
       // B operand (k = r, column bj): B_z = sum_z' mf * psi_z'(r, b)
-      double pz[5];
+      double pz[NZ];
       const size_t ob = (size_t)(ib0 + (bcol ? bj : 0)) * G + (rok ? r : 0);
 #pragma unroll
-      for (int z = 0; z < 5; ++z) pz[z] = (z < nz && bcol && rok) ? P.psi[z][ob] : 0.0;
-      double Bre[5], Bim[5];
+      for (int z = 0; z < NZ; ++z) pz[z] = (bcol && rok) ? P.psi[z][ob] : 0.0;
+      double Bre[NZ], Bim[NZ];
 #pragma unroll
-      for (int z = 0; z < 5; ++z) {
+      for (int z = 0; z < NZ; ++z) {
         double br = 0.0, bi = 0.0;
 #pragma unroll
-        for (int zp = 0; zp < 5; ++zp) {
-          br += sMF[rr][sb][z][zp][0] * pz[zp];
-          bi += sMF[rr][sb][z][zp][1] * pz[zp];
+        for (int zp = 0; zp < NZ; ++zp) {
+          br += sMF[rr][sb][z * 5 + zp][0] * pz[zp];
+          bi += sMF[rr][sb][z * 5 + zp][1] * pz[zp];
         }
         Bre[z] = br;
         Bim[z] = bi;
       }
       // A operand (row a, k = r) and the products
 #pragma unroll
-      for (int q = 0; q < P2_MAXRT; ++q) {
-        if (q >= nrt) break;
+      for (int q = 0; q < NRT; ++q) {
         const int arow = q * 16 + mi;
         const bool aok = arow < nrows && rok;
         const size_t oa = (size_t)(ia + t.a_lo + (aok ? arow : 0)) * G + (rok ? r : 0);
 #pragma unroll
-        for (int z = 0; z < 5; ++z) {
-          if (z >= nz) break;
+        for (int z = 0; z < NZ; ++z) {
           const double av = aok ? P.psi[z][oa] : 0.0;
           C[q][0] = mfma64(av, Bre[z], C[q][0]);
           C[q][1] = mfma64(av, Bim[z], C[q][1]);
@@ -944,8 +946,7 @@ __global__ void __launch_bounds__(64) k_project_mfma(ProjArgs P, const Tile2 *__
   double *oim = ore + P.nxy;
   const int bcolo = t.b0 + (lane & 15);
 #pragma unroll
-  for (int q = 0; q < P2_MAXRT; ++q) {
-    if (q >= nrt) break;
+  for (int q = 0; q < NRT; ++q) {
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       const int arow = q * 16 + 4 * v + (lane >> 4);
@@ -1294,6 +1295,7 @@ struct pnfam_gpu_ctx {
   int pair_off[4] = {0, 0, 0, 0}, pair_n[4] = {0, 0, 0, 0};
   DevPtr d_tiles, d_tiles2;
   int ntiles = 0, ntiles2 = 0;
+  int pj_first[16] = {}, pj_count[16] = {};   // projection tiles per kind * 8 + row tiles
   bool basic_kernels = false;   // PNFAM_GPU_BASIC=1: the plain FP64 kernels (cross-check path)
   // per-frequency workspace
   int cap = 0;
@@ -1555,7 +1557,26 @@ int stage_hamiltonian(pnfam_gpu_ctx *h, int na, const int *act) {
     hipLaunchKernelGGL(k_project, dim3(h->ntiles, na), dim3(256), 0, h->stream, P);
   } else if (h->ntiles2) {
     Prof pr(h, K_PROJECT, na);
-    hipLaunchKernelGGL(k_project_mfma, dim3(h->ntiles2, na), dim3(64), 0, h->stream, P, h->d_tiles2.as<Tile2>());
+    const Tile2 *tl = h->d_tiles2.as<Tile2>();
+    for (int k = 0; k < 16; ++k) {
+      const int n = h->pj_count[k];
+      if (!n) continue;
+      const dim3 grid(n, na);
+      const Tile2 *tk = tl + h->pj_first[k];
+#define PJ_LAUNCH(NRT, KIND) hipLaunchKernelGGL((k_project_mfma<NRT, KIND>), grid, dim3(64), 0, h->stream, P, tk)
+      switch (k) {
+        case 1: PJ_LAUNCH(1, 0); break;
+        case 2: PJ_LAUNCH(2, 0); break;
+        case 3: PJ_LAUNCH(3, 0); break;
+        case 4: PJ_LAUNCH(4, 0); break;
+        case 9: PJ_LAUNCH(1, 1); break;
+        case 10: PJ_LAUNCH(2, 1); break;
+        case 11: PJ_LAUNCH(3, 1); break;
+        case 12: PJ_LAUNCH(4, 1); break;
+        default: return fail(PNFAM_GPU_EINVAL, "projection tile class out of range");
+      }
+#undef PJ_LAUNCH
+    }
   }
   HIP_TRY(hipGetLastError());
   return 0;
@@ -2129,6 +2150,15 @@ int pnfam_gpu_set_operator(pnfam_gpu_ctx *h, int32_t beta_minus, const int32_t *
           }
       }
     }
+  }
+  // order by (kind, row tiles) for the templated projection kernel
+  auto tkey = [](const Tile2 &t) { return t.kind * 8 + (t.a_hi - t.a_lo + 15) / 16; };
+  std::stable_sort(tiles2.begin(), tiles2.end(), [&](const Tile2 &a, const Tile2 &b) { return tkey(a) < tkey(b); });
+  for (int k = 0; k < 16; ++k) { h->pj_first[k] = 0; h->pj_count[k] = 0; }
+  for (size_t i = 0; i < tiles2.size(); ++i) {
+    const int k = tkey(tiles2[i]);
+    if (h->pj_count[k] == 0) h->pj_first[k] = (int)i;
+    h->pj_count[k]++;
   }
   h->ntiles2 = (int)tiles2.size();
   if ((e = upload(h->d_tiles2, tiles2.data(), std::max<size_t>(1, tiles2.size()) * sizeof(Tile2), h->stream)))
