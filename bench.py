@@ -1,16 +1,22 @@
-"""Throughput of the batched pnFAM iteration on one MI355X.
+"""Throughput of the batched pnFAM iteration on one MI355X, mixed vs FP64.
 
-A step solves one batch of contour frequencies to convergence for each of
-GT- K=0 and K=1 on the N_sh=16, Z=60 N=92, beta2=0.3, 40x40-grid workload
-(synthetic, structurally exact HFB input; see pynfam_amd/synthetic.py).
-value = converged frequency points per second over the timed steps (a point
-counts when its max-abs residual drops below 1e-7; points reaching max_iter
-count as failures).  Inputs are resident in HBM before timing starts.
+A step solves, for each of GT- K=0 and K=1, the computed half (128 points) of
+pynfam's CIRCLE contour on [0, EQRPA_max] to convergence, on the N_sh=16,
+Z=60 N=92, beta2=0.3, 40x40-grid workload (synthetic, structurally exact HFB
+input; see pynfam_amd/synthetic.py).  The two K batches run from two host
+threads at once unless --concurrent 0.  value = converged frequency points per
+second of the mixed-precision path over the timed steps (a point counts when
+its max-abs residual drops below 1e-7; points reaching max_iter count as
+failures); the FP64 path runs the same steps and is reported beside it.
+Inputs are resident in HBM before timing starts.
 
-The JSON line also carries the dominant kernel's achieved rate against the
-MI355X FP64 peak (HIP events around every launch on the engine stream during
-the timed steps) and the CPU oracle timed on a bounded sample of the same
-workload on this host.
+The JSON line also carries:
+- the relative strength error and the allowed-GT rate error of the mixed path
+  against the FP64 path on the same contour;
+- the dominant kernel region's achieved rate against the MI355X peak of the
+  arithmetic it runs (HIP events around every launch on the engine stream
+  during the timed steps);
+- the CPU oracle timed on a bounded sample of the same workload on this host.
 """
 import argparse
 import json

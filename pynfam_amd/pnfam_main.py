@@ -73,8 +73,9 @@ def build_from_namelist(nml, hel_path):
     if int(ext["two_body_current_mode"]) != 0:
         raise ValueError("two-body currents are not supported by this engine")
     op = str(ext["operator_name"]).strip().upper()
-    if op not in ("F", "GT"):
-        raise ValueError(f"operator {op!r} is not supported by this engine (F, GT)")
+    from .extfield import PARITY_EVEN
+    if op not in PARITY_EVEN:
+        raise ValueError(f"operator {op!r} is not supported by this engine ({', '.join(PARITY_EVEN)})")
     return build_problem(hel_path, op, int(ext["operator_k"]), str(ext["beta_type"]).strip(),
                          str(inter["interaction_name"]).strip(),
                          quench_residual_int=float(solv["quench_residual_int"]),
@@ -133,7 +134,7 @@ def write_header(log, nml, prob, parameter_filename, precision_name):
     log.line()
 
 
-def write_result(log, omega, strength, iterations, si, max_iter, seconds):
+def write_result(log, omega, strength, iterations, si, max_iter, seconds, crossterms=()):
     log.line("-" * 72)
     if iterations >= 0:
         log.line(f" *  FAM iteration converged after {iterations:6d} steps.  si = {si:12.5E}")
@@ -148,6 +149,8 @@ def write_result(log, omega, strength, iterations, si, max_iter, seconds):
     log.line(f"{'Real':>44s}{'Imag':>34s}", raw=True)
     log.line(f" {'Energy':<10s}{omega.real:34.19E}{omega.imag:34.19E}", raw=True)
     log.line(f" {'Strength':<10s}{strength.real:34.19E}{strength.imag:34.19E}", raw=True)
+    for lbl, v in crossterms:
+        log.line(f" {lbl:<10s}{v.real:34.19E}{v.imag:34.19E}", raw=True)
 
 
 def run(parameter_filename, workdir="."):
@@ -162,12 +165,27 @@ def run(parameter_filename, workdir="."):
         prob = build_from_namelist(nml, os.path.join(workdir, HEL_FILE))
         write_header(log, nml, prob, parameter_filename, prec_name)
         omega = complex(float(g["real_eqrpa"]), float(g["imag_eqrpa"]))
+        want_x = bool(nml["ext_field"]["compute_crossterms"])
+        xts = []
+        if want_x:
+            from .crossterms import crossterm_problems
+            xts = crossterm_problems(prob)
+            log.line("Cross-terms computed:          " + (", ".join(l for l, _ in xts) or "None"))
         eng = GpuEngine(prob)
         r = eng.solve([omega], max_iter=int(solv["max_iter"]), eps=float(solv["convergence_epsilon"]),
-                      broyden_history=int(solv["broyden_history_size"]), precision=prec)
+                      broyden_history=int(solv["broyden_history_size"]), precision=prec, want_x=bool(xts))
         eng.close()
+        cross = []
+        if xts:
+            from .crossterms import contract
+            # the operator's own contraction leads the list, as the reference prints it
+            for lbl, gp in xts:
+                ge = GpuEngine(gp)
+                g20, g02 = ge.fqp()
+                ge.close()
+                cross.append((lbl, complex(contract(r["x"], g20, g02)[0])))
         write_result(log, omega, complex(r["strength"][0]), int(r["iterations"][0]), float(r["si"][0]),
-                     int(solv["max_iter"]), time.process_time() - t0)
+                     int(solv["max_iter"]), time.process_time() - t0, cross)
         return 0
     except Exception as exc:   # the reference's abort: message, then a normal stop
         log.line(f" ERROR: {exc}")

@@ -29,6 +29,8 @@ SETS = {
     "S40": "tests/S40 closed tests/pynfam_test_GT-S40/000000",
     "Gd162": "tests/Gd162 closed tests/GT/000000",
 }
+# every one-body operator (F, GT and the first-forbidden set) on the S40 solution
+ALL_OPS = {"S40_All": "tests/S40_GT_All/000000"}
 
 FLOAT = r"[-+]?\d+\.\d+(?:[EeDd][-+]?\d+)?"
 
@@ -82,6 +84,10 @@ def parse_dat(txt):
         m = re.match(r"\s+Strength\s+(\S+)\s+(\S+)", line)
         if m:
             rec["strength"] = [_f(m.group(1)), _f(m.group(2))]
+            continue
+        m = re.match(r"\s+(\w+x\w+)\s+(" + FLOAT + r")\s+(" + FLOAT + r")\s*$", line)
+        if m:
+            rec.setdefault("crossterms", {})[m.group(1)] = [_f(m.group(2)), _f(m.group(3))]
             continue
         m = re.match(r"# Number shells:\s+(\d+)", line)
         if m:
@@ -223,12 +229,48 @@ def extract_rates(name, rel):
     print(name, "rates", sorted(rates))
 
 
+def extract_points_only(name, rel, every=6):
+    """HFB input and every `every`-th archived point of each operator-K (the
+    full traces are kept; cross-terms included)."""
+    src = os.path.join(REF, rel)
+    dst = os.path.join(HERE, name)
+    os.makedirs(dst, exist_ok=True)
+    for fn in ("hfbtho_NAMELIST.dat", "hfbtho_output.hel"):
+        shutil.copyfile(os.path.join(src, "hfb_soln", fn), os.path.join(dst, fn))
+    meta = os.path.join(src, "fam_soln", "fam_meta")
+    out = {}
+    for tname in sorted(os.listdir(meta)):
+        if not tname.endswith(".tar"):
+            continue
+        op = tname[:-4]
+        points = {}
+        with tarfile.open(os.path.join(meta, tname)) as tf:
+            for member in tf.getmembers():
+                if not member.isfile():
+                    continue
+                pid = member.name.split("/")[1]
+                if int(pid) % every:
+                    continue
+                txt = tf.extractfile(member).read().decode()
+                d = points.setdefault(pid, {})
+                if member.name.endswith(".in"):
+                    d["namelist"] = parse_nml_text(txt)
+                elif member.name.endswith(".dat"):
+                    d.update(parse_dat(txt))
+        out[op] = [dict(point=k, **v) for k, v in sorted(points.items())]
+        print(name, op, len(out[op]), "points")
+    with open(os.path.join(dst, "points.json"), "w") as fh:
+        json.dump(out, fh, indent=0)
+
+
 def main():
     if not os.path.isdir(REF):
         sys.exit("reference tree not present")
     for name, rel in SETS.items():
         extract_set(name, rel)
         extract_rates(name, rel)
+    for name, rel in ALL_OPS.items():
+        extract_points_only(name, rel)
 
 
 if __name__ == "__main__":

@@ -70,7 +70,7 @@ def test_dat_format_parses(tmp_path):
 
 def test_errors_end_in_log_with_status_zero(tmp_path):
     p = tmp_path / "bad.in"
-    p.write_text("&general fam_output_filename = 'bad' /\n&ext_field operator_name = 'RS2' /\n")
+    p.write_text("&general fam_output_filename = 'bad' /\n&ext_field operator_name = 'XYZ' /\n")
     assert pnfam_main.run(str(p), workdir=str(tmp_path)) == 0
     txt = (tmp_path / "bad.dat").read_text()
     assert "ERROR" in txt and "Strength" not in txt

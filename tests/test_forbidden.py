@@ -1,0 +1,87 @@
+"""First-forbidden operators (R, P, RS0, RS1, RS2, PS0) and their cross-terms,
+plus F and GT, on the archived S40 run with every operator (tests/S40_GT_All in
+the reference): oracle strengths, iteration counts and cross-terms against the
+archived values, and the cross-term contraction against the strength itself."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+
+SET = os.path.join(GOLDEN, "S40_All")
+
+
+def _points():
+    with open(os.path.join(SET, "points.json")) as fh:
+        return json.load(fh)
+
+
+_PROBS = {}
+
+
+def all_problem(op, K):
+    from pynfam_amd.problem import build_problem
+    key = (op, K)
+    if key not in _PROBS:
+        nml = _points()[f"{op}-K{K}"][0]["namelist"]
+        _PROBS[key] = build_problem(os.path.join(SET, "hfbtho_output.hel"), op, K, nml["beta_type"],
+                                    nml["interaction_name"], vpair_t0=nml["vpair_t0"],
+                                    override_cs0=nml["override_cs0"], override_csr=nml["override_csr"],
+                                    override_cds=nml["override_cds"])
+    return _PROBS[key]
+
+
+OPK = [("F", 0), ("GT", 0), ("GT", 1), ("R", 0), ("R", 1), ("P", 0), ("P", 1), ("RS0", 0), ("RS1", 0),
+       ("RS1", 1), ("RS2", 0), ("RS2", 1), ("RS2", 2), ("PS0", 0)]
+
+
+@pytest.mark.parametrize("op,K", OPK)
+def test_oracle_strengths_and_crossterms(op, K):
+    from oracle.oracle import Oracle
+    from pynfam_amd.crossterms import contract, crossterm_problems
+    prob = all_problem(op, K)
+    orc = Oracle(prob)
+    pts = _points()[f"{op}-K{K}"][:2]
+    for p in pts:
+        w = complex(*p["energy"])
+        o = orc.ifam(w)
+        ref = complex(*p["strength"])
+        assert o["iterations"] == p["iterations"], (op, K, o["iterations"], p["iterations"])
+        assert abs(o["strength"] - ref) < 1e-9 * abs(ref), (op, K, o["strength"], ref)
+        # the contraction with the operator's own Fqp reproduces S
+        f20, f02, _, _ = orc.fqp()
+        assert abs(contract(o["x"], f20, f02)[0] - o["strength"]) < 1e-12 * abs(ref)
+        xts = crossterm_problems(prob)
+        assert sorted(lbl for lbl, _ in xts) == sorted(p.get("crossterms", {}).keys())
+        for lbl, gp in xts:
+            g20, g02, _, _ = Oracle(gp).fqp()
+            got = contract(o["x"], g20, g02)[0]
+            want = complex(*p["crossterms"][lbl])
+            assert abs(got - want) < 1e-9 * max(abs(want), 1e-3 * abs(ref)), (lbl, got, want)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("op,K", OPK)
+def test_gpu_strengths_and_crossterms(op, K):
+    """All archived points of the set in one batch on the GPU engine; cross-terms
+    from the returned amplitudes and the cross operators' Fqp from the engine."""
+    from pynfam_amd.crossterms import contract, crossterm_problems
+    from pynfam_amd.engine import GpuEngine
+    prob = all_problem(op, K)
+    pts = _points()[f"{op}-K{K}"]
+    w = np.array([complex(*p["energy"]) for p in pts])
+    eng = GpuEngine(prob)
+    r = eng.solve(w, want_x=True)
+    S = np.array([complex(*p["strength"]) for p in pts])
+    assert np.max(np.abs(r["strength"] - S) / np.abs(S)) < 1e-7
+    assert np.all(r["iterations"] == np.array([p["iterations"] for p in pts]))
+    for lbl, gp in crossterm_problems(prob):
+        ge = GpuEngine(gp)
+        g20, g02 = ge.fqp()
+        ge.close()
+        got = contract(r["x"], g20, g02)
+        want = np.array([complex(*p["crossterms"][lbl]) for p in pts])
+        assert np.max(np.abs(got - want) / np.maximum(np.abs(want), 1e-3 * np.abs(S))) < 1e-7, lbl
+    eng.close()
