@@ -96,7 +96,8 @@ def measured_traffic(kernel):
 
 
 # peak of the arithmetic each kernel issues (MI355X dense figures)
-KERNEL_PEAK = {"mix_gemm_density": (2500.0, "TFLOP/s"), "mix_gemm_project": (2500.0, "TFLOP/s")}
+KERNEL_PEAK = {"mix_gemm_density": (2500.0, "TFLOP/s"), "mix_gemm_project": (2500.0, "TFLOP/s"),
+               "fac_density": (2500.0, "TFLOP/s"), "fac_project": (2500.0, "TFLOP/s")}
 
 
 def measure(engines, w, precision, args):

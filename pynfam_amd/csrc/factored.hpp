@@ -1,0 +1,478 @@
+// Factored split-BF16 grid stage of the mixed-precision path (device side).
+//
+// The same two contractions of calc_hamiltonian (pnfam_hamiltonian_blas.f90:
+// 124-711 densities, 1105-1262 projection) as the FP64 kernels, in the factored
+// order of the reference, with BF16 MFMA (v_mfma_f32_16x16x32_bf16) on hi + lo
+// split operands (three products hi*hi + hi*lo + lo*hi, FP32 accumulation):
+//
+//   densities   T_x^s(b, r) = sum_{a of spin s} rho(a, b) psi_x(r, a)   (MFMA)
+//               D(r)       += f(T(b, r), psi_y(r, b))                   (FP32 vector)
+//   projection  h(a, b)     = sum_{z, r} psi_z(r, a) B_z(r, b)          (MFMA)
+//               B_z(r, b)   = sum_z' mf(z, z', s_a, s_b)(r) psi_z'(r, b) (FP32 vector)
+//
+// Work per frequency is that of the FP64 kernels: no pair tables are streamed,
+// so a pass over few frequencies costs little.  The omega-independent psi
+// operands are BF16 hi/lo tables built once per operator:
+//   PA[z][state][RP]             projection rows (k = grid point, contiguous)
+//   DB[x][chunk][RP][32]         density columns: chunks of 32 states of one
+//                                spin of one block, zero-padded
+// The per-frequency rho/kappa operand is converted once per pass into the same
+// chunking (FR, one 32-state chunk per (pair, column b, spin of a, chunk)).
+//
+// MFMA operand layout (16x16x32, wave64): lane i + 16 g supplies A[i][8g..8g+7]
+// and B[8g..8g+7][i]; the result D[4 g + v][i] sits in lane i + 16 g, register v.
+
+constexpr int FAC_CH = 32;    // states per chunk (one MFMA K step)
+constexpr int FAC_RT = 32;    // grid points per projection K step
+
+struct FacBlock {             // chunk bases and counts of the two spins of a block
+  int cb0, nc0, cb1, nc1;
+};
+
+// ---- omega-independent tables --------------------------------------------
+// PA[z][state][RP]: psi_z(r, state), zero for r >= nghl.
+__global__ void k_fac_ptab(const double *__restrict__ psi0, const double *__restrict__ psi1,
+                           const double *__restrict__ psi2, const double *__restrict__ psi3,
+                           const double *__restrict__ psi4, int nst, int nghl, int RP, __bf16 *__restrict__ hi,
+                           __bf16 *__restrict__ lo) {
+  const int z = blockIdx.y;
+  const double *p = z == 0 ? psi0 : z == 1 ? psi1 : z == 2 ? psi2 : z == 3 ? psi3 : psi4;
+  const size_t n = (size_t)nst * RP;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+    const int r = (int)(e % RP);
+    const size_t st = e / RP;
+    const float v = r < nghl ? (float)p[st * nghl + r] : 0.0f;
+    __bf16 h, l;
+    split_bf16(v, h, l);
+    hi[(size_t)z * n + e] = h;
+    lo[(size_t)z * n + e] = l;
+  }
+}
+
+// DB[x][chunk][RP][32]: psi_x(r, state0(chunk) + j) for j < nvalid(chunk).
+// chunk_info[c] = (first state, valid states).
+__global__ void k_fac_dtab(const double *__restrict__ psi0, const double *__restrict__ psi1,
+                           const double *__restrict__ psi2, const double *__restrict__ psi3,
+                           const int2 *__restrict__ chunk_info, int nchunk, int nghl, int RP, __bf16 *__restrict__ hi,
+                           __bf16 *__restrict__ lo) {
+  const int x = blockIdx.y;
+  const double *p = x == 0 ? psi0 : x == 1 ? psi1 : x == 2 ? psi2 : psi3;
+  const size_t n = (size_t)nchunk * RP * FAC_CH;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+    const int j = (int)(e % FAC_CH);
+    const int r = (int)((e / FAC_CH) % RP);
+    const int c = (int)(e / ((size_t)FAC_CH * RP));
+    const int2 ci = chunk_info[c];
+    const float v = (j < ci.y && r < nghl) ? (float)p[(size_t)(ci.x + j) * nghl + r] : 0.0f;
+    __bf16 h, l;
+    split_bf16(v, h, l);
+    hi[(size_t)x * n + e] = h;
+    lo[(size_t)x * n + e] = l;
+  }
+}
+
+// ---- per-pass operand conversion -------------------------------------------
+// One segment = the states of one spin of block ix in one column b of a pair:
+// src = element offset of its first state in the quadrant, n = its length,
+// dst = offset of its first chunk in FR, q = quadrant (rsp slot) it reads.
+struct FacSeg {
+  int src, n, dst, q;
+};
+
+// FR[slot][ri][L] (hi and lo separately) from rsp[w][8][nxy]; four segments
+// per workgroup of 256 threads, one wave each.
+__global__ void __launch_bounds__(256) k_fac_rho(const double *__restrict__ rsp, const FacSeg *__restrict__ segs,
+                                                 int nseg, long nxy, long L, const int *__restrict__ act,
+                                                 __bf16 *__restrict__ hi, __bf16 *__restrict__ lo) {
+  const int sidx = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (sidx >= nseg) return;
+  const int lane = threadIdx.x & 63;
+  const int slot = blockIdx.y;
+  const int w = act[slot];
+  const FacSeg sg = segs[sidx];
+  const int len = (sg.n + FAC_CH - 1) / FAC_CH * FAC_CH;
+  for (int ri = 0; ri < 2; ++ri) {
+    const double *src = rsp + ((size_t)w * 8 + 2 * sg.q + ri) * nxy + sg.src;
+    const size_t o = ((size_t)slot * 2 + ri) * L + sg.dst;
+    for (int j = lane; j < len; j += 64) {
+      const float v = j < sg.n ? (float)src[j] : 0.0f;
+      __bf16 h, l;
+      split_bf16(v, h, l);
+      hi[o + j] = h;
+      lo[o + j] = l;
+    }
+  }
+}
+
+// ---- densities ---------------------------------------------------------------
+struct FacDensArgs {
+  const __bf16 *FRh, *FRl;        // [slot][ri][L]
+  long L;
+  const __bf16 *DBh, *DBl;        // [x][chunk][RP][32]
+  long dbx;                       // elements per x plane of DB
+  const FacBlock *blk;
+  const int4 *pairs;              // (ix, iy, FR offset, 0) per pair
+  int rho_first[2], rho_n[2], kap_first[2], kap_n[2];   // pair ranges per sector
+  const double *wf, *wfdr, *wfdp, *wfdz, *wdcori;
+  const int *isstart, *db, *nsu;
+  int nghl, RP;
+  double *dens;                   // [w][2][56][nghl]
+  const int *act;
+  int n3;                         // active slots below n3 take three BF16 products
+  int psplit;                     // workgroups sharing the pairs of one (r tile, slot, sector)
+  float *part;                    // psplit > 1: unscaled partial densities [part][slot][sec][56][nghl]
+};
+
+__device__ inline v4f mfma_bf16(v8bf a, v8bf b, v4f c) { return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
+
+// acc += A * B with the split products: hi*hi, hi*lo, lo*hi (three) or hi*hi.
+__device__ inline v4f mfma_split(v8bf ah, v8bf al, v8bf bh, v8bf bl, v4f c, bool three) {
+  c = mfma_bf16(ah, bh, c);
+  if (three) {
+    c = mfma_bf16(ah, bl, c);
+    c = mfma_bf16(al, bh, c);
+  }
+  return c;
+}
+
+constexpr int FAC_NW = 4;     // waves per workgroup of the factored kernels
+
+// One workgroup of FAC_NW waves per (16 grid points, frequency, sector, part);
+// the block pairs are dealt round-robin to the FAC_NW x psplit waves of the
+// psplit parts, so a frequency spreads over that many waves.  The waves of a
+// workgroup are summed through LDS; with psplit > 1 the parts go to a partial
+// buffer that k_fac_dsum adds up in a fixed order.
+// Lane i + 16 g owns grid point r0 + i and the columns b0 + 4 g + v of each
+// 16-column tile; column tiles never straddle the spin boundary of block iy,
+// so the spin of b is uniform over a tile.
+__global__ void __launch_bounds__(64 * FAC_NW, 2) k_fac_density(FacDensArgs A) {
+  __shared__ float sRed[FAC_NW][2 * NDENS][16];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int fi = lane & 15, fg = lane >> 4;
+  const int r0 = blockIdx.x * 16;
+  const int slot = blockIdx.y;
+  const int w = A.act[slot];
+  const int sec = blockIdx.z & 1, prt = blockIdx.z >> 1;
+  const int G = A.nghl;
+  const int r = r0 + fi;
+  const bool rok = r < G;
+  const int rc = rok ? r : 0;
+  const bool three = slot < A.n3;
+  const int pstep = FAC_NW * A.psplit, poff = wave + FAC_NW * prt;
+  float acc[2 * NDENS];
+#pragma unroll
+  for (int d = 0; d < 2 * NDENS; ++d) acc[d] = 0.0f;
+  const __bf16 *frh[2] = {A.FRh + ((size_t)slot * 2 + 0) * A.L, A.FRh + ((size_t)slot * 2 + 1) * A.L};
+  const __bf16 *frl[2] = {A.FRl + ((size_t)slot * 2 + 0) * A.L, A.FRl + ((size_t)slot * 2 + 1) * A.L};
+  const double *tab[4] = {A.wf, A.wfdr, A.wfdp, A.wfdz};
+  const v4f z4 = {0.f, 0.f, 0.f, 0.f};
+
+  // mean-field pairs: x = 0..3
+  for (int p = A.rho_first[sec] + poff; p < A.rho_first[sec] + A.rho_n[sec]; p += pstep) {
+    const int4 pr = A.pairs[p];
+    const int ix = pr.x, iy = pr.y;
+    const FacBlock bx = A.blk[ix];
+    const int nchT = bx.nc0 + bx.nc1;
+    const int dy = A.db[iy], ny = A.nsu[iy], ib0 = A.isstart[iy];
+    for (int sb = 0; sb < 2; ++sb) {
+      const int bbeg = sb ? ny : 0, bend = sb ? dy : ny;
+      const int s = sb ? -1 : 1;
+      for (int b0 = bbeg; b0 < bend; b0 += 16) {
+        v4f T[4][2][2];
+#pragma unroll
+        for (int x = 0; x < 4; ++x)
+#pragma unroll
+          for (int sa = 0; sa < 2; ++sa) { T[x][sa][0] = z4; T[x][sa][1] = z4; }
+        const int brow = min(b0 + fi, bend - 1);          // A operand row (column b of rho)
+#pragma unroll
+        for (int sa = 0; sa < 2; ++sa) {
+          const int nc = sa ? bx.nc1 : bx.nc0;
+          const int cb = sa ? bx.cb1 : bx.cb0;
+          const size_t arow = (size_t)pr.z + ((size_t)brow * nchT + (sa ? bx.nc0 : 0)) * FAC_CH + 8 * fg;
+          for (int c = 0; c < nc; ++c) {
+            const size_t ao = arow + (size_t)c * FAC_CH;
+            const v8bf ah0 = *(const v8bf *)(frh[0] + ao), al0 = *(const v8bf *)(frl[0] + ao);
+            const v8bf ah1 = *(const v8bf *)(frh[1] + ao), al1 = *(const v8bf *)(frl[1] + ao);
+            const size_t bo = ((size_t)(cb + c) * A.RP + r0 + fi) * FAC_CH + 8 * fg;
+#pragma unroll
+            for (int x = 0; x < 4; ++x) {
+              const v8bf bh = *(const v8bf *)(A.DBh + (size_t)x * A.dbx + bo);
+              const v8bf bl = *(const v8bf *)(A.DBl + (size_t)x * A.dbx + bo);
+              T[x][sa][0] = mfma_split(ah0, al0, bh, bl, T[x][sa][0], three);
+              T[x][sa][1] = mfma_split(ah1, al1, bh, bl, T[x][sa][1], three);
+            }
+          }
+        }
+        // epilogue: lane holds T(b = b0 + 4 fg + v, r); spin of b is s
+        const int sd = sb, so = 1 - sb;   // T index of a's spin equal / opposite to b's
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          if (b0 + 4 * fg + v >= bend || !rok) break;
+          const size_t oy = (size_t)(ib0 + b0 + 4 * fg + v) * G + rc;
+          float y[4];
+#pragma unroll
+          for (int yy = 0; yy < 4; ++yy) y[yy] = (float)tab[yy][oy];
+          float Pr[4][4], Pi[4][4], Qr[4][4], Qi[4][4];
+#pragma unroll
+          for (int x = 0; x < 4; ++x)
+#pragma unroll
+            for (int yy = 0; yy < 4; ++yy) {
+              Pr[x][yy] = T[x][sd][0][v] * y[yy]; Pi[x][yy] = T[x][sd][1][v] * y[yy];
+              Qr[x][yy] = T[x][so][0][v] * y[yy]; Qi[x][yy] = T[x][so][1][v] * y[yy];
+            }
+          density_accumulate(acc, Pr, Pi, Qr, Qi, s);
+        }
+      }
+    }
+  }
+  // pairing densities from kappa (x = 0)
+  for (int p = A.kap_first[sec] + poff; p < A.kap_first[sec] + A.kap_n[sec]; p += pstep) {
+    const int4 pr = A.pairs[p];
+    const int ix = pr.x, iy = pr.y;
+    const FacBlock bx = A.blk[ix];
+    const int nchT = bx.nc0 + bx.nc1;
+    const int dy = A.db[iy], ny = A.nsu[iy], ib0 = A.isstart[iy];
+    for (int sb = 0; sb < 2; ++sb) {
+      const int bbeg = sb ? ny : 0, bend = sb ? dy : ny;
+      const int s = sb ? -1 : 1;
+      for (int b0 = bbeg; b0 < bend; b0 += 16) {
+        float wbv[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) wbv[v] = (float)A.wf[(size_t)(ib0 + min(b0 + 4 * fg + v, bend - 1)) * G + rc];
+        v4f T[2][2];
+#pragma unroll
+        for (int sa = 0; sa < 2; ++sa) { T[sa][0] = z4; T[sa][1] = z4; }
+        const int brow = min(b0 + fi, bend - 1);
+#pragma unroll
+        for (int sa = 0; sa < 2; ++sa) {
+          const int nc = sa ? bx.nc1 : bx.nc0;
+          const int cb = sa ? bx.cb1 : bx.cb0;
+          const size_t arow = (size_t)pr.z + ((size_t)brow * nchT + (sa ? bx.nc0 : 0)) * FAC_CH + 8 * fg;
+          for (int c = 0; c < nc; ++c) {
+            const size_t ao = arow + (size_t)c * FAC_CH;
+            const v8bf ah0 = *(const v8bf *)(frh[0] + ao), al0 = *(const v8bf *)(frl[0] + ao);
+            const v8bf ah1 = *(const v8bf *)(frh[1] + ao), al1 = *(const v8bf *)(frl[1] + ao);
+            const size_t bo = ((size_t)(cb + c) * A.RP + r0 + fi) * FAC_CH + 8 * fg;
+            const v8bf bh = *(const v8bf *)(A.DBh + bo);
+            const v8bf bl = *(const v8bf *)(A.DBl + bo);
+            T[sa][0] = mfma_split(ah0, al0, bh, bl, T[sa][0], three);
+            T[sa][1] = mfma_split(ah1, al1, bh, bl, T[sa][1], three);
+          }
+        }
+        const int sd = sb, so = 1 - sb;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          if (b0 + 4 * fg + v >= bend || !rok) break;
+          const float wb = wbv[v];
+          float pr_ = 2.0f * T[so][0][v] * wb, pi_ = 2.0f * T[so][1][v] * wb;
+          acc[2 * D_RB] -= s * pr_; acc[2 * D_RB + 1] -= s * pi_;
+          acc[2 * D_SBZ] += pr_; acc[2 * D_SBZ + 1] += pi_;
+          pr_ = 2.0f * T[sd][0][v] * wb; pi_ = 2.0f * T[sd][1][v] * wb;
+          acc[2 * D_SBR] += s < 0 ? pr_ : -pr_; acc[2 * D_SBR + 1] += s < 0 ? pi_ : -pi_;
+          acc[2 * D_SBP] += pi_; acc[2 * D_SBP + 1] -= pr_;
+        }
+      }
+    }
+  }
+  // sum the lane groups of each wave, then the waves
+#pragma unroll
+  for (int d = 0; d < 2 * NDENS; ++d) {
+    float v = acc[d];
+    v += __shfl_xor(v, 16, 64);
+    v += __shfl_xor(v, 32, 64);
+    if (fg == 0) sRed[wave][d][fi] = v;
+  }
+  __syncthreads();
+  double *out = A.dens + (((size_t)w * 2 + sec) * 2 * NDENS) * G;
+  float *pout = A.part + ((((size_t)prt * gridDim.y + slot) * 2 + sec) * 2 * NDENS) * G;
+  for (int e = threadIdx.x; e < 2 * NDENS * 16; e += 64 * FAC_NW) {
+    const int d = e >> 4, i = e & 15;
+    const int rr = r0 + i;
+    if (rr >= G) continue;
+    float v = 0.0f;
+#pragma unroll
+    for (int j = 0; j < FAC_NW; ++j) v += sRed[j][d][i];
+    if (A.psplit == 1) out[(size_t)d * G + rr] = (double)v * A.wdcori[rr];
+    else pout[(size_t)d * G + rr] = v;
+  }
+}
+
+// dens[w][sec][d][r] = wdcori[r] * sum over the psplit parts (in part order).
+__global__ void k_fac_dsum(const float *__restrict__ part, int psplit, int na, int nghl,
+                           const double *__restrict__ wdcori, const int *__restrict__ act, double *__restrict__ dens) {
+  const int slot = blockIdx.y, sec = blockIdx.z;
+  const int w = act[slot];
+  const size_t plane = (size_t)2 * NDENS * nghl;
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < 2 * NDENS * nghl; e += gridDim.x * blockDim.x) {
+    const int r = e % nghl;
+    float v = 0.0f;
+    for (int q = 0; q < psplit; ++q) v += part[(((size_t)q * na + slot) * 2 + sec) * plane + e];
+    dens[((size_t)w * 2 + sec) * plane + e] = (double)v * wdcori[r];
+  }
+}
+
+// ---- projection -------------------------------------------------------------
+struct FacProjArgs {
+  const __bf16 *PAh, *PAl;        // [z][state][RP]
+  long pz;                        // elements per z plane of PA
+  const double *psi[5];           // FP64 [state][nghl]
+  const int *isstart, *db, *nsu;
+  const double *flds;             // [w][2][208][nghl]
+  double *hsp;                    // [w][8][nxy]
+  long nxy;
+  int nghl, RP;
+  const int *act;
+  int n3;                         // active slots below n3 take three BF16 products
+};
+
+// The z' of each z with a non-zero mean field (pnfam_hamiltonian_blas.f90:
+// 774-1093): z = 0 couples to z' = 0..4, z = 1..3 to z' = 0..3, z = 4 to z' = 0.
+// Combos in the order (z, z') of the LDS field array.
+__host__ __device__ constexpr int fac_zp_count(int z) { return z == 0 ? 5 : (z == 4 ? 1 : 4); }
+__host__ __device__ constexpr int fac_zp_first(int z) { return z == 0 ? 0 : (z == 4 ? 17 : 5 + 4 * (z - 1)); }
+
+// One workgroup of FAC_NW waves per (projection tile, frequency): rows NRT x 16
+// states of spin s_a of block ix, 16 columns of block iy, K = (z, r).  Wave j
+// takes the 32-point steps j, j + FAC_NW, ...; each wave stages the fields of
+// its step in its own LDS slice (FP32) and builds B_z(r, b) for its column b
+// and 8 grid points per lane.  The waves' partial results are summed through LDS.
+template <int NRT, int KIND>
+__global__ void __launch_bounds__(64 * FAC_NW, 2) k_fac_project(FacProjArgs P, const Tile2 *__restrict__ tiles) {
+  constexpr int NZ = KIND == 0 ? 5 : 1;
+  constexpr int NC = KIND == 0 ? 18 : 1;
+  constexpr int STG = FAC_RT * 2 * NC * 2;            // staged floats per wave and step
+  constexpr int NPL = (STG + 63) / 64;                // per lane
+  constexpr int RED = (FAC_NW - 1) * NRT * 8 * 64;    // reduction floats
+  constexpr int SM = FAC_NW * STG > RED ? FAC_NW * STG : RED;
+  __shared__ __attribute__((aligned(16))) float smem[SM];
+  const Tile2 t = tiles[blockIdx.x];
+  const int w = P.act[blockIdx.y];
+  const int G = P.nghl;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int fi = lane & 15, fg = lane >> 4;
+  float *sMF = smem + wave * STG;                      // [r][s_b][combo][ri]
+  const int ia = P.isstart[t.ix], ib0 = P.isstart[t.iy];
+  const int dx = P.db[t.ix], dy = P.db[t.iy];
+  const int nrows = t.a_hi - t.a_lo;
+  const bool three = (int)blockIdx.y < P.n3;
+  const double *fl = P.flds + (((size_t)w * 2 + t.sec) * 2 * NFLD) * G;
+  const int bj = min(t.b0 + fi, dy - 1);           // this lane's column (clamped; outputs masked)
+  const int sb = bj < P.nsu[t.iy] ? 0 : 1;
+  const double *pcol[NZ];
+#pragma unroll
+  for (int z = 0; z < NZ; ++z) pcol[z] = P.psi[z] + (size_t)(ib0 + bj) * G;
+  int arow[NRT];
+#pragma unroll
+  for (int q = 0; q < NRT; ++q) arow[q] = ia + t.a_lo + min(q * 16 + fi, nrows - 1);
+  // staged value k of this lane: grid point rr, re/im ri, combo k % NC, spin of b k / NC
+  static_assert(STG % 64 == 0, "whole staging rounds");
+  const int rr = lane & (FAC_RT - 1), ri_l = (lane >> 5) & 1;
+  auto field_of = [&](int k) {      // k is a compile-time constant after unrolling
+    const int cc = k % NC, sbb = k / NC;
+    if (KIND == 0) {
+      int z = 0, zp = cc;
+      if (cc >= 17) { z = 4; zp = 0; }
+      else if (cc >= 5) { z = 1 + (cc - 5) / 4; zp = (cc - 5) % 4; }
+      return mf_index(z, zp, t.sa, sbb);
+    }
+    return 100 + (sbb == 0 ? (t.sa == 0 ? 0 : 1) : (t.sa == 0 ? 2 : 3));
+  };
+  const double *flane = fl + (size_t)ri_l * G + rr;
+  auto fetch = [&](int rc, float (&v)[NPL]) {
+    const bool ok = rc + rr < G;
+#pragma unroll
+    for (int k = 0; k < NPL; ++k) v[k] = ok ? (float)flane[(size_t)(2 * field_of(k)) * G + rc] : 0.0f;
+  };
+  v4f C[NRT][2];
+  const v4f z4 = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int q = 0; q < NRT; ++q) { C[q][0] = z4; C[q][1] = z4; }
+  for (int rc = wave * FAC_RT; rc < P.RP; rc += FAC_NW * FAC_RT) {
+    // this step's fields into the wave's slice
+    {
+      float cur[NPL];
+      fetch(rc, cur);
+#pragma unroll
+      for (int k = 0; k < NPL; ++k) sMF[((rr * 2 + k / NC) * NC + k % NC) * 2 + ri_l] = cur[k];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const int rb = rc + 8 * fg;
+    float pz[NZ][8];
+#pragma unroll
+    for (int z = 0; z < NZ; ++z)
+#pragma unroll
+      for (int i = 0; i < 8; ++i) pz[z][i] = (float)pcol[z][min(rb + i, G - 1)];
+    // per z: B_z for this lane's column and 8 grid points, split, then the
+    // products with every row tile
+#pragma unroll
+    for (int z = 0; z < NZ; ++z) {
+      v8bf Bh0, Bl0, Bh1, Bl1;
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        float br = 0.f, bi = 0.f;
+#pragma unroll
+        for (int k = 0; k < (KIND == 0 ? fac_zp_count(z) : 1); ++k) {
+          const int cc = KIND == 0 ? fac_zp_first(z) + k : 0;
+          const int zp = KIND == 0 ? (z == 4 ? 0 : k) : 0;
+          const float2 m = *(const float2 *)&sMF[(((8 * fg + i) * 2 + sb) * NC + cc) * 2];
+          br += m.x * pz[zp][i];
+          bi += m.y * pz[zp][i];
+        }
+        __bf16 h, l;
+        split_bf16(br, h, l); Bh0[i] = h; Bl0[i] = l;
+        split_bf16(bi, h, l); Bh1[i] = h; Bl1[i] = l;
+      }
+#pragma unroll
+      for (int q = 0; q < NRT; ++q) {
+        const size_t ao = (size_t)z * P.pz + (size_t)arow[q] * P.RP + rb;
+        const v8bf ah = *(const v8bf *)(P.PAh + ao), al = *(const v8bf *)(P.PAl + ao);
+        C[q][0] = mfma_split(ah, al, Bh0, Bl0, C[q][0], three);
+        C[q][1] = mfma_split(ah, al, Bh1, Bl1, C[q][1], three);
+      }
+    }
+    // the slice is rewritten next step: every lane's reads come first
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+  // sum the waves: waves 1.. park their tiles in LDS, wave 0 adds and writes
+  __syncthreads();
+  if (wave > 0) {
+    float *red = smem + (size_t)(wave - 1) * NRT * 8 * 64;
+#pragma unroll
+    for (int q = 0; q < NRT; ++q)
+#pragma unroll
+      for (int ri = 0; ri < 2; ++ri)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) red[((q * 2 + ri) * 4 + v) * 64 + lane] = C[q][ri][v];
+  }
+  __syncthreads();
+  if (wave != 0) return;
+#pragma unroll
+  for (int j = 0; j < FAC_NW - 1; ++j) {
+    const float *red = smem + (size_t)j * NRT * 8 * 64;
+#pragma unroll
+    for (int q = 0; q < NRT; ++q)
+#pragma unroll
+      for (int ri = 0; ri < 2; ++ri)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) C[q][ri][v] += red[((q * 2 + ri) * 4 + v) * 64 + lane];
+  }
+  double *ore = P.hsp + ((size_t)w * 8 + 2 * t.out_slot) * P.nxy + t.off;
+  double *oim = ore + P.nxy;
+  const int bcol = t.b0 + fi;
+#pragma unroll
+  for (int q = 0; q < NRT; ++q) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int ar = q * 16 + 4 * fg + v;
+      if (ar < nrows && bcol < dy) {
+        const size_t o = (size_t)(t.a_lo + ar) + (size_t)bcol * dx;
+        ore[o] = 2.0 * (double)C[q][0][v];
+        oim[o] = 2.0 * (double)C[q][1][v];
+      }
+    }
+  }
+}

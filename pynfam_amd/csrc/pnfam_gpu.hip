@@ -12,11 +12,13 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstdlib>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <numeric>
 #include <string>
 #include <vector>
 
@@ -405,56 +407,57 @@ __global__ void __launch_bounds__(64) k_density(DensArgs A) {
 // psi_y(b) (pnfam_hamiltonian_blas.f90:255-581).
 // P[x][y] = sum over a of the spin of b, Q[x][y] = sum over a of the other
 // spin, of psi_x(a) rho(a, b) psi_y(b), for columns b of spin s (+1/-1).
-template <class R>
-__device__ inline void density_accumulate(double *acc, const R (&Pr)[4][4], const R (&Pi)[4][4],
+template <class R, class A>
+__device__ inline void density_accumulate(A *acc, const R (&Pr)[4][4], const R (&Pi)[4][4],
                                           const R (&Qr)[4][4], const R (&Qi)[4][4], int s) {
 #define ACC(d, ri) acc[2 * (d) + (ri)]
-  double ar, ai;
+  const A half = A(0.5);
+  A ar, ai;
   ACC(D_RHO, 0) += Pr[0][0]; ACC(D_RHO, 1) += Pi[0][0];
   ACC(D_SZ, 0) += s * Pr[0][0]; ACC(D_SZ, 1) += s * Pi[0][0];
-  ar = 0.5 * (Pr[0][2] + Pr[2][0]); ai = 0.5 * (Pi[0][2] + Pi[2][0]);
+  ar = half * (Pr[0][2] + Pr[2][0]); ai = half * (Pi[0][2] + Pi[2][0]);
   ACC(D_JP, 0) += ar; ACC(D_JP, 1) += ai;
   ACC(D_TJPZ, 0) += s * ar; ACC(D_TJPZ, 1) += s * ai;
   ar = Pr[1][1] + Pr[2][2] + Pr[3][3]; ai = Pi[1][1] + Pi[2][2] + Pi[3][3];
   ACC(D_TAU, 0) += ar; ACC(D_TAU, 1) += ai;
   ACC(D_TZ, 0) += s * ar; ACC(D_TZ, 1) += s * ai;
-  ar = 0.5 * (Pr[0][1] - Pr[1][0]); ai = 0.5 * (Pi[0][1] - Pi[1][0]);
+  ar = half * (Pr[0][1] - Pr[1][0]); ai = half * (Pi[0][1] - Pi[1][0]);
   ACC(D_JR, 0) -= ai; ACC(D_JR, 1) += ar;
   ACC(D_TJRZ, 0) -= s * ai; ACC(D_TJRZ, 1) += s * ar;
-  ar = 0.5 * (Pr[0][3] - Pr[3][0]); ai = 0.5 * (Pi[0][3] - Pi[3][0]);
+  ar = half * (Pr[0][3] - Pr[3][0]); ai = half * (Pi[0][3] - Pi[3][0]);
   ACC(D_JZ, 0) -= ai; ACC(D_JZ, 1) += ar;
   ACC(D_TJZZ, 0) -= s * ai; ACC(D_TJZZ, 1) += s * ar;
   ACC(D_GS, 0) += s * (Pr[0][3] + Pr[3][0]); ACC(D_GS, 1) += s * (Pi[0][3] + Pi[3][0]);
-  ACC(D_FR, 0) += 0.5 * s * (Pr[1][3] + Pr[3][1]); ACC(D_FR, 1) += 0.5 * s * (Pi[1][3] + Pi[3][1]);
+  ACC(D_FR, 0) += half * s * (Pr[1][3] + Pr[3][1]); ACC(D_FR, 1) += half * s * (Pi[1][3] + Pi[3][1]);
   ar = Pr[2][3] - Pr[3][2]; ai = Pi[2][3] - Pi[3][2];
-  ACC(D_FP, 0) -= 0.5 * s * ai; ACC(D_FP, 1) += 0.5 * s * ar;
+  ACC(D_FP, 0) -= half * s * ai; ACC(D_FP, 1) += half * s * ar;
   ACC(D_FZ, 0) += s * Pr[3][3]; ACC(D_FZ, 1) += s * Pi[3][3];
-  const double e = -(double)s;
+  const A e = A(-s);
   ACC(D_SR, 0) += Qr[0][0]; ACC(D_SR, 1) += Qi[0][0];
   ACC(D_SP, 0) -= e * Qi[0][0]; ACC(D_SP, 1) += e * Qr[0][0];
-  const double t2r = 0.5 * (Qr[2][0] + Qr[0][2]), t2i = 0.5 * (Qi[2][0] + Qi[0][2]);
+  const A t2r = half * (Qr[2][0] + Qr[0][2]), t2i = half * (Qi[2][0] + Qi[0][2]);
   ACC(D_TJPR, 0) += t2r; ACC(D_TJPR, 1) += t2i;
   ACC(D_TJPP, 0) -= e * t2i; ACC(D_TJPP, 1) += e * t2r;
   ACC(D_GS, 0) += e * (Qr[0][2] - Qr[2][0]) + (Qr[0][1] + Qr[1][0]);
   ACC(D_GS, 1) += e * (Qi[0][2] - Qi[2][0]) + (Qi[0][1] + Qi[1][0]);
-  const double tkr = Qr[1][1] + Qr[2][2] + Qr[3][3], tki = Qi[1][1] + Qi[2][2] + Qi[3][3];
+  const A tkr = Qr[1][1] + Qr[2][2] + Qr[3][3], tki = Qi[1][1] + Qi[2][2] + Qi[3][3];
   ACC(D_TR, 0) += tkr; ACC(D_TR, 1) += tki;
   ACC(D_TP, 0) -= e * tki; ACC(D_TP, 1) += e * tkr;
-  const double jrr = 0.5 * (Qr[0][1] - Qr[1][0]), jri = 0.5 * (Qi[0][1] - Qi[1][0]);
+  const A jrr = half * (Qr[0][1] - Qr[1][0]), jri = half * (Qi[0][1] - Qi[1][0]);
   ACC(D_TJRR, 0) -= jri; ACC(D_TJRR, 1) += jrr;
   ACC(D_TJRP, 0) -= e * jrr; ACC(D_TJRP, 1) -= e * jri;
-  const double jzr = 0.5 * (Qr[0][3] - Qr[3][0]), jzi = 0.5 * (Qi[0][3] - Qi[3][0]);
+  const A jzr = half * (Qr[0][3] - Qr[3][0]), jzi = half * (Qi[0][3] - Qi[3][0]);
   ACC(D_TJZR, 0) -= jzi; ACC(D_TJZR, 1) += jzr;
   ACC(D_TJZP, 0) -= e * jzr; ACC(D_TJZP, 1) -= e * jzi;
-  ar = Qr[1][1] + e * 0.5 * (Qr[1][2] - Qr[2][1]);
-  ai = Qi[1][1] + e * 0.5 * (Qi[1][2] - Qi[2][1]);
+  ar = Qr[1][1] + e * half * (Qr[1][2] - Qr[2][1]);
+  ai = Qi[1][1] + e * half * (Qi[1][2] - Qi[2][1]);
   ACC(D_FR, 0) += ar; ACC(D_FR, 1) += ai;
-  ar = e * Qr[2][2] + 0.5 * (Qr[2][1] - Qr[1][2]);
-  ai = e * Qi[2][2] + 0.5 * (Qi[2][1] - Qi[1][2]);
+  ar = e * Qr[2][2] + half * (Qr[2][1] - Qr[1][2]);
+  ai = e * Qi[2][2] + half * (Qi[2][1] - Qi[1][2]);
   ACC(D_FP, 0) -= ai; ACC(D_FP, 1) += ar;
   ar = Qr[3][1] + Qr[1][3] + e * (Qr[3][2] - Qr[2][3]);
   ai = Qi[3][1] + Qi[1][3] + e * (Qi[3][2] - Qi[2][3]);
-  ACC(D_FZ, 0) += 0.5 * ar; ACC(D_FZ, 1) += 0.5 * ai;
+  ACC(D_FZ, 0) += half * ar; ACC(D_FZ, 1) += half * ai;
 #undef ACC
 }
 
@@ -959,6 +962,8 @@ __global__ void __launch_bounds__(64) k_project_mfma(ProjArgs P, const Tile2 *__
   }
 }
 
+#include "factored.hpp"
+
 struct GreenArgs {
   const double *hqp;     // [w][4][nxy]
   const double *f20, *f02;
@@ -1123,7 +1128,9 @@ __global__ void k_bro_norm(BroArgs B) {
   B.scal[(size_t)w * 4 + 0] = (B.iter >= 2 && B.M > 0) ? 1.0 / sqrt(s) : 1.0;
 }
 
-// scale the history slot, then partial dots df_j . df_ipos and df_j . vout
+// scale the history slot, then partial dots df_j . df_ipos and df_j . vout,
+// eight history vectors per pass over the new difference and vout
+constexpr int BRO_JB = 8;
 __global__ void __launch_bounds__(BRO_WG) k_bro_dots(BroArgs B) {
   __shared__ double red[BRO_WG / 64];
   const int w = B.act[blockIdx.y];
@@ -1136,17 +1143,30 @@ __global__ void __launch_bounds__(BRO_WG) k_bro_dots(BroArgs B) {
   if (B.iter >= 2)
     for (long e = e0; e < N; e += es) { dfp[e] *= normi; dvp[e] *= normi; }
   double *pp = B.part + ((size_t)w * B.nwg + blockIdx.x) * (2 * BRO_MAXM + 2) + 2;
-  for (int j = 0; j < B.iter_used; ++j) {
-    const double *dfj = B.df + ((size_t)w * B.M + j) * N;
-    double g = 0.0, v = 0.0;
+  for (int j0 = 0; j0 < B.iter_used; j0 += BRO_JB) {
+    const int nj = min(BRO_JB, B.iter_used - j0);
+    double g[BRO_JB], v[BRO_JB];
+#pragma unroll
+    for (int j = 0; j < BRO_JB; ++j) { g[j] = 0.0; v[j] = 0.0; }
+    const double *df0 = B.df + ((size_t)w * B.M + j0) * N;
     for (long e = e0; e < N; e += es) {
-      const double a = dfj[e];
-      g += a * dfp[e];
-      v += a * vo[e];
+      const double p = dfp[e], o = vo[e];
+#pragma unroll
+      for (int j = 0; j < BRO_JB; ++j) {
+        if (j < nj) {
+          const double a = df0[(size_t)j * N + e];
+          g[j] += a * p;
+          v[j] += a * o;
+        }
+      }
     }
-    const double sg = wg_sum(g, red);
-    const double sv = wg_sum(v, red);
-    if (threadIdx.x == 0) { pp[2 * j] = sg; pp[2 * j + 1] = sv; }
+#pragma unroll
+    for (int j = 0; j < BRO_JB; ++j) {
+      if (j >= nj) break;
+      const double sg = wg_sum(g[j], red);
+      const double sv = wg_sum(v[j], red);
+      if (threadIdx.x == 0) { pp[2 * (j0 + j)] = sg; pp[2 * (j0 + j) + 1] = sv; }
+    }
   }
 }
 
@@ -1323,6 +1343,14 @@ struct pnfam_gpu_ctx {
   bool own_gemm = false;
   DevPtr d_Oproj;   // projection results [Kp][ncol] of the largest set (FP32)
   DevPtr d_Psplit;  // split-K partial planes (FP32)
+  // factored split-BF16 grid stage: psi tables, chunk maps, pair lists, operand segments
+  int mix_grid = 0;             // 0 auto, 1 pair tables, 2 factored (PNFAM_GPU_MIX_GRID)
+  int mix_auto_n = 6;           // auto: factored at or below this many active points
+  bool fac_ready = false;
+  DevPtr PAh, PAl, DBh, DBl, d_fac_blk, d_fac_pairs, d_fac_segs, FRh, FRl, d_fac_dpart;
+  long fac_L = 0, fac_pz = 0, fac_dbx = 0;
+  int fac_nseg = 0, fac_cap = 0;
+  int fac_first[4] = {0, 0, 0, 0}, fac_n[4] = {0, 0, 0, 0};   // rho0, kap0, rho1, kap1 pair ranges
   int unit_precision = PNFAM_GPU_FP64;   // grid stage of the per-unit entry points
   int bro_nwg = 0;
   double flops_q2s = 0, flops_s2q = 0;
@@ -1346,11 +1374,12 @@ namespace {
 
 enum KernelId { K_TRIPROD_Q2S, K_DENSITY, K_FIELDS, K_PROJECT, K_TRIPROD_S2Q, K_GREEN, K_BRO_RESID, K_BRO_NORM,
                 K_BRO_DOTS, K_BRO_SOLVE, K_BRO_UPDATE, K_CONTRACT, K_CONTRACT_FIN, K_ZERO, K_MIXACC,
-                K_MIX_CVT, K_MIX_GEMM_D, K_MIX_DENS, K_MIX_GEMM_P, K_NKERNEL };
+                K_MIX_CVT, K_MIX_GEMM_D, K_MIX_DENS, K_MIX_GEMM_P, K_FAC_CVT, K_FAC_DENS, K_FAC_PROJ, K_NKERNEL };
 const char *kKernelNames[K_NKERNEL] = {"triprod_qp2sp", "density", "fields", "project", "triprod_sp2qp", "green",
                                        "bro_resid", "bro_norm", "bro_dots", "bro_solve", "bro_update", "contract",
                                        "contract_fin", "zero_slots", "mix_accumulate", "mix_convert",
-                                       "mix_gemm_density", "mix_densities", "mix_gemm_project"};
+                                       "mix_gemm_density", "mix_densities", "mix_gemm_project", "fac_convert",
+                                       "fac_density", "fac_project"};
 
 hipEvent_t pool_event(pnfam_gpu_ctx *h) {
   if (!h->event_pool.empty()) { hipEvent_t e = h->event_pool.back(); h->event_pool.pop_back(); return e; }
@@ -1457,7 +1486,7 @@ int ensure_workspace(pnfam_gpu_ctx *h, int npts, int M) {
   if ((e = h->d_hqp.alloc(n * 4 * nxy * 8))) return e;
   if ((e = h->d_dens.alloc(n * 2 * 2 * NDENS * (size_t)h->nghl * 8))) return e;
   if ((e = h->d_flds.alloc(n * 2 * 2 * NFLD * (size_t)h->nghl * 8))) return e;
-  h->bro_nwg = (int)std::min<long>(64, std::max<long>(1, (long)(4 * nxy + BRO_WG - 1) / BRO_WG));
+  h->bro_nwg = (int)std::min<long>(256, std::max<long>(1, (long)(4 * nxy + BRO_WG - 1) / BRO_WG));
   if ((e = h->d_part.alloc(n * h->bro_nwg * (2 * BRO_MAXM + 2) * 8))) return e;
   if ((e = h->d_cpart.alloc(n * 64 * 2 * 8))) return e;
   if ((e = h->d_S.alloc(n * 2 * 8))) return e;
@@ -1646,6 +1675,8 @@ int mix_build_set(pnfam_gpu_ctx *h, pnfam_gpu_ctx::MixSet &S, const BlockMap &m,
   return 0;
 }
 
+int fac_prepare(pnfam_gpu_ctx *h);
+
 int mix_prepare(pnfam_gpu_ctx *h) {
   if (h->mix_ready) return 0;
   if (!same_pairs(h->rspmap[0], h->fmap) || !same_pairs(h->rspmap[3], h->fmap))
@@ -1654,6 +1685,19 @@ int mix_prepare(pnfam_gpu_ctx *h) {
     return fail(PNFAM_GPU_EINVAL, "mixed path: kappa structure differs from Delta");
   h->RP = (h->nghl + MIX_RALIGN - 1) / MIX_RALIGN * MIX_RALIGN;
   if (const char *ev = std::getenv("PNFAM_GPU_MIX_GEMM")) h->own_gemm = std::strcmp(ev, "own") == 0;
+  h->mix_grid = 0;
+  if (const char *ev = std::getenv("PNFAM_GPU_MIX_GRID")) {
+    if (std::strcmp(ev, "tables") == 0) h->mix_grid = 1;
+    else if (std::strcmp(ev, "factored") == 0) h->mix_grid = 2;
+    else if (std::strcmp(ev, "auto") != 0) return fail(PNFAM_GPU_EINVAL, "PNFAM_GPU_MIX_GRID must be tables, factored or auto");
+  }
+  if (const char *ev = std::getenv("PNFAM_GPU_MIX_AUTO_N")) h->mix_auto_n = std::atoi(ev);
+  if (h->mix_grid != 1)
+    if (int e = fac_prepare(h)) return e;
+  if (h->mix_grid == 2) {   // factored only: no pair tables
+    h->mix_ready = true;
+    return 0;
+  }
   if (!h->own_gemm && !h->blas) {
     if (rocblas_create_handle(&h->blas) != rocblas_status_success)
       return fail(PNFAM_GPU_EDEVICE, "rocblas_create_handle failed");
@@ -1666,13 +1710,13 @@ int mix_prepare(pnfam_gpu_ctx *h) {
   if ((e = mix_build_set(h, h->mix[2], h->hspmap[2], 1, 1))) return e;
   h->mix_ready = true;
   h->mix_cap = 0;
-  // MFMA-issued FLOP per frequency (three BF16 products per split product,
-  // padded element and grid ranges included)
+  // MFMA-issued FLOP per frequency and BF16 product (a pass issues one or
+  // three products; padded element and grid ranges included)
   double fd = 0, fp = 0;
   for (int si = 0; si < 3; ++si) {
     const double cols = si == 0 ? 4.0 : 2.0;
-    fd += 3.0 * 2.0 * h->mix[si].ntd * (double)h->RP * h->mix[si].Kp * cols;
-    fp += 3.0 * 2.0 * h->mix[si].ntp * (double)h->RP * h->mix[si].Kp * cols;
+    fd += 2.0 * h->mix[si].ntd * (double)h->RP * h->mix[si].Kp * cols;
+    fp += 2.0 * h->mix[si].ntp * (double)h->RP * h->mix[si].Kp * cols;
   }
   h->flops_unit[K_MIX_GEMM_D] = fd;
   h->flops_unit[K_MIX_GEMM_P] = fp;
@@ -1738,13 +1782,15 @@ int blas_nt_chunks(pnfam_gpu_ctx *h, int64_t ns, int64_t nt, int64_t k, int64_t 
   return 0;
 }
 
-// The three split products hi*hi + hi*lo + lo*hi of one contraction.
+// The split products of one contraction: hi*hi over all ns columns of the
+// small operand, hi*lo + lo*hi over its first ns3 columns (the points that
+// take three products come first in the active list).
 int blas_split(pnfam_gpu_ctx *h, int64_t ns, int64_t nt, int64_t k, const __bf16 *Sh, const __bf16 *Sl, int64_t lds,
-               const __bf16 *Th, const __bf16 *Tl, int64_t ldt, float *C, int64_t ldc, int nprod) {
+               const __bf16 *Th, const __bf16 *Tl, int64_t ldt, float *C, int64_t ldc, int64_t ns3) {
   if (int e = blas_nt(h, ns, nt, k, Sh, lds, Th, ldt, C, ldc, false)) return e;
-  if (nprod == 1) return 0;
-  if (int e = blas_nt(h, ns, nt, k, Sh, lds, Tl, ldt, C, ldc, true)) return e;
-  return blas_nt(h, ns, nt, k, Sl, lds, Th, ldt, C, ldc, true);
+  if (ns3 == 0) return 0;
+  if (int e = blas_nt(h, ns3, nt, k, Sh, lds, Tl, ldt, C, ldc, true)) return e;
+  return blas_nt(h, ns3, nt, k, Sl, lds, Th, ldt, C, ldc, true);
 }
 
 // Split-K form of blas_split for contractions with few output tiles (the pairing
@@ -1761,17 +1807,19 @@ __global__ void k_sum_planes(const float *__restrict__ P, int nplanes, size_t pl
 }
 
 int blas_split_k(pnfam_gpu_ctx *h, int64_t ns, int64_t nt, int64_t k, const __bf16 *Sh, const __bf16 *Sl, int64_t lds,
-                 const __bf16 *Th, const __bf16 *Tl, int64_t ldt, float *C, int64_t ldc, int nprod, float *P,
+                 const __bf16 *Th, const __bf16 *Tl, int64_t ldt, float *C, int64_t ldc, int64_t ns3, float *P,
                  size_t pcap) {
   const int64_t nfull = k / MIX_KCHUNK, rem = k - nfull * MIX_KCHUNK;
   const int64_t plane = nt * ldc;
   if ((size_t)(nfull + 1) * (size_t)plane > pcap) return fail(PNFAM_GPU_EINVAL, "split-K planes exceed their buffer");
   const __bf16 *Ss[3] = {Sh, Sh, Sl}, *Ts[3] = {Th, Tl, Th};
-  for (int p = 0; p < nprod; ++p) {
+  for (int p = 0; p < 3; ++p) {
+    const int64_t nsp = p == 0 ? ns : ns3;
+    if (nsp == 0) break;
     if (nfull > 0)
-      if (int e = blas_nt_chunks(h, ns, nt, MIX_KCHUNK, nfull, Ss[p], lds, Ts[p], ldt, P, ldc, plane, p > 0)) return e;
+      if (int e = blas_nt_chunks(h, nsp, nt, MIX_KCHUNK, nfull, Ss[p], lds, Ts[p], ldt, P, ldc, plane, p > 0)) return e;
     if (rem > 0)
-      if (int e = blas_nt(h, ns, nt, rem, Ss[p] + nfull * MIX_KCHUNK, lds, Ts[p] + nfull * MIX_KCHUNK, ldt,
+      if (int e = blas_nt(h, nsp, nt, rem, Ss[p] + nfull * MIX_KCHUNK, lds, Ts[p] + nfull * MIX_KCHUNK, ldt,
                           P + nfull * plane, ldc, p > 0))
         return e;
   }
@@ -1790,7 +1838,8 @@ void mix_gemm(pnfam_gpu_ctx *h, GemmArgs G, const pnfam_gpu_ctx::MixSet &S, int 
 }
 
 // calc_hamiltonian for the active columns on split-BF16 MFMA: rsp -> hsp.
-int stage_hamiltonian_mixed(pnfam_gpu_ctx *h, int na, const int *act, int nprod = 3) {
+// n3: the first n3 active points take three BF16 products, the rest one.
+int stage_hamiltonian_mixed(pnfam_gpu_ctx *h, int na, const int *act, int n3) {
   if (int e = mix_workspace(h, h->cap)) return e;
   const size_t RP = h->RP;
   const int cap = h->mix_cap;
@@ -1808,7 +1857,7 @@ int stage_hamiltonian_mixed(pnfam_gpu_ctx *h, int na, const int *act, int nprod 
     } }
   // 2. density contractions
   if (!h->own_gemm) {
-    Prof pr(h, K_MIX_GEMM_D, na);
+    Prof pr(h, K_MIX_GEMM_D, na + 2 * n3);
     for (int si = 0; si < 3; ++si) {
       auto &S = h->mix[si];
       const long ncol = (si == 0 ? 4 : 2) * (long)cap;
@@ -1820,18 +1869,19 @@ int stage_hamiltonian_mixed(pnfam_gpu_ctx *h, int na, const int *act, int nprod 
           hipLaunchKernelGGL(k_fill_f32, dim3(1024), dim3(256), 0, h->stream, O, (size_t)nT * ncol);
           continue;
         }
+        const int64_t nS3 = (si == 0 ? 4 : 2) * (int64_t)n3;
         if (si != 0) {   // pairing sets: one table row per grid point, too few output tiles
           if (int e = blas_split_k(h, nS, nT, kl, S.Bh.as<__bf16>() + k0, S.Bl.as<__bf16>() + k0, S.Kp,
-                                   S.TDh.as<__bf16>() + k0, S.TDl.as<__bf16>() + k0, S.Kp, O, ncol, nprod,
+                                   S.TDh.as<__bf16>() + k0, S.TDl.as<__bf16>() + k0, S.Kp, O, ncol, nS3,
                                    h->d_Psplit.as<float>(), h->d_Psplit.bytes / 4))
             return e;
         } else if (int e = blas_split(h, nS, nT, kl, S.Bh.as<__bf16>() + k0, S.Bl.as<__bf16>() + k0, S.Kp,
-                                      S.TDh.as<__bf16>() + k0, S.TDl.as<__bf16>() + k0, S.Kp, O, ncol, nprod))
+                                      S.TDh.as<__bf16>() + k0, S.TDl.as<__bf16>() + k0, S.Kp, O, ncol, nS3))
           return e;
       }
     }
   } else {
-    Prof pr(h, K_MIX_GEMM_D, na);
+    Prof pr(h, K_MIX_GEMM_D, na + 2 * n3);
     for (int si = 0; si < 3; ++si) {
       auto &S = h->mix[si];
       const long ncol = (si == 0 ? 4 : 2) * (long)cap;
@@ -1865,7 +1915,7 @@ int stage_hamiltonian_mixed(pnfam_gpu_ctx *h, int na, const int *act, int nprod 
   // 6. projections: rocBLAS into O[k][n] and a scatter into the FP64 block
   // matrices, or the hand-written kernel writing them directly
   if (!h->own_gemm) {
-    Prof pr(h, K_MIX_GEMM_P, na);
+    Prof pr(h, K_MIX_GEMM_P, na + 2 * n3);
     for (int si = 0; si < 3; ++si) {
       auto &S = h->mix[si];
       const long ncol = (si == 0 ? 4 : 2) * (long)cap;
@@ -1877,7 +1927,7 @@ int stage_hamiltonian_mixed(pnfam_gpu_ctx *h, int na, const int *act, int nprod 
         const size_t fo = (size_t)c * ncol * KK;
         if (int e = blas_split(h, nS, kl, KK, S.Fh.as<__bf16>() + fo, S.Fl.as<__bf16>() + fo, KK,
                                S.TPh.as<__bf16>() + (size_t)k0 * KK, S.TPl.as<__bf16>() + (size_t)k0 * KK, KK,
-                               O + (size_t)k0 * ncol, ncol, nprod))
+                               O + (size_t)k0 * ncol, ncol, (si == 0 ? 4 : 2) * (int64_t)n3))
           return e;
       }
       hipLaunchKernelGGL(k_mix_scatter, dim3((unsigned)std::min(2048, (S.Kp + 3) / 4), (unsigned)((nS + 63) / 64)),
@@ -1885,7 +1935,7 @@ int stage_hamiltonian_mixed(pnfam_gpu_ctx *h, int na, const int *act, int nprod 
                          si == 0 ? 0 : (si == 1 ? 1 : 2), 3, act, h->d_hsp.as<double>(), h->nxy);
     }
   } else {
-    Prof pr(h, K_MIX_GEMM_P, na);
+    Prof pr(h, K_MIX_GEMM_P, na + 2 * n3);
     for (int si = 0; si < 3; ++si) {
       auto &S = h->mix[si];
       const long ncol = (si == 0 ? 4 : 2) * (long)cap;
@@ -1900,6 +1950,201 @@ int stage_hamiltonian_mixed(pnfam_gpu_ctx *h, int na, const int *act, int nprod 
     } }
   HIP_TRY(hipGetLastError());
   return 0;
+}
+
+// ---- factored split-BF16 grid stage (mixed path) -----------------------------
+// psi tables, chunk map of every (block, spin), the density pair lists with
+// their operand offsets and the conversion segments of the four density
+// structures (rho_pn, kappa+, rho_np, kappa- as the FP64 density kernel reads them).
+int fac_prepare(pnfam_gpu_ctx *h) {
+  if (h->fac_ready) return 0;
+  const int nb = (int)h->db.size();
+  const int RP = h->RP;
+  // spin order inside every block: states [0, nsu) up, [nsu, d) down
+  for (int i = 0; i < nb; ++i)
+    for (int j = 0; j < h->db[i]; ++j)
+      if ((h->ns[h->isstart[i] + j] > 0) != (j < h->nsu[i]))
+        return fail(PNFAM_GPU_EINVAL, "factored grid stage: block states are not ordered spin up, spin down");
+  std::vector<FacBlock> blk(nb);
+  std::vector<int2> chunks;
+  for (int i = 0; i < nb; ++i) {
+    const int n0 = h->nsu[i], n1 = h->db[i] - h->nsu[i];
+    blk[i].nc0 = (n0 + FAC_CH - 1) / FAC_CH;
+    blk[i].nc1 = (n1 + FAC_CH - 1) / FAC_CH;
+    blk[i].cb0 = (int)chunks.size();
+    for (int c = 0; c < blk[i].nc0; ++c) chunks.push_back(make_int2(h->isstart[i] + c * FAC_CH, std::min(FAC_CH, n0 - c * FAC_CH)));
+    blk[i].cb1 = (int)chunks.size();
+    for (int c = 0; c < blk[i].nc1; ++c)
+      chunks.push_back(make_int2(h->isstart[i] + n0 + c * FAC_CH, std::min(FAC_CH, n1 - c * FAC_CH)));
+  }
+  const int nchunk = std::max<int>(1, (int)chunks.size());
+  if (chunks.empty()) chunks.push_back(make_int2(0, 0));
+  // pairs and segments: structure order rho0 (m11), kap0 (m12), rho1 (m22), kap1 (m21)
+  const int quad[4] = {0, 1, 3, 2};
+  std::vector<int4> pairs;
+  std::vector<FacSeg> segs;
+  long L = 0;
+  for (int st = 0; st < 4; ++st) {
+    h->fac_first[st] = (int)pairs.size();
+    for (const auto &pq : pair_list(h->rspmap[quad[st]])) {
+      const int ix = pq.x, iy = pq.y, dx = h->db[ix], dy = h->db[iy];
+      const int nchT = blk[ix].nc0 + blk[ix].nc1;
+      pairs.push_back(make_int4(ix, iy, (int)L, 0));
+      for (int b = 0; b < dy; ++b)
+        for (int sa = 0; sa < 2; ++sa) {
+          const int n = sa ? dx - h->nsu[ix] : h->nsu[ix];
+          if (n == 0) continue;
+          FacSeg g;
+          g.src = pq.z + b * dx + (sa ? h->nsu[ix] : 0);
+          g.n = n;
+          g.dst = (int)(L + ((long)b * nchT + (sa ? blk[ix].nc0 : 0)) * FAC_CH);
+          g.q = quad[st];
+          segs.push_back(g);
+        }
+      L += (long)dy * nchT * FAC_CH;
+    }
+    h->fac_n[st] = (int)pairs.size() - h->fac_first[st];
+  }
+  if (L >= (1L << 31)) return fail(PNFAM_GPU_EINVAL, "factored grid stage: operand too large");
+  h->fac_L = std::max<long>(L, FAC_CH);
+  h->fac_nseg = (int)segs.size();
+  int e;
+  if ((e = upload(h->d_fac_blk, blk.data(), std::max<size_t>(1, blk.size()) * sizeof(FacBlock), h->stream)) ||
+      (e = upload(h->d_fac_pairs, pairs.data(), std::max<size_t>(1, pairs.size()) * sizeof(int4), h->stream)) ||
+      (e = upload(h->d_fac_segs, segs.data(), std::max<size_t>(1, segs.size()) * sizeof(FacSeg), h->stream)))
+    return e;
+  DevPtr d_chunks;
+  if ((e = upload(d_chunks, chunks.data(), chunks.size() * sizeof(int2), h->stream))) return e;
+  const int nst = (int)std::accumulate(h->db.begin(), h->db.end(), 0);
+  h->fac_pz = (long)nst * RP;
+  h->fac_dbx = (long)nchunk * RP * FAC_CH;
+  if ((e = h->PAh.alloc(5 * h->fac_pz * 2)) || (e = h->PAl.alloc(5 * h->fac_pz * 2)) ||
+      (e = h->DBh.alloc(4 * h->fac_dbx * 2)) || (e = h->DBl.alloc(4 * h->fac_dbx * 2)))
+    return e;
+  hipLaunchKernelGGL(k_fac_ptab, dim3(2048, 5), dim3(256), 0, h->stream, h->d_wf.as<double>(), h->d_wfdr.as<double>(),
+                     h->d_wfdp.as<double>(), h->d_wfdz.as<double>(), h->d_wfd2.as<double>(), nst, h->nghl, RP,
+                     h->PAh.as<__bf16>(), h->PAl.as<__bf16>());
+  hipLaunchKernelGGL(k_fac_dtab, dim3(2048, 4), dim3(256), 0, h->stream, h->d_wf.as<double>(), h->d_wfdr.as<double>(),
+                     h->d_wfdp.as<double>(), h->d_wfdz.as<double>(), d_chunks.as<int2>(), nchunk, h->nghl, RP,
+                     h->DBh.as<__bf16>(), h->DBl.as<__bf16>());
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(h->stream));   // d_chunks is freed on return
+  // MFMA-issued FLOP per frequency and BF16 product: density tiles of 16 grid
+  // points x 16 columns per chunk (4 tables mean field, 1 pairing, re + im);
+  // projection tiles of NRT x 16 rows x 16 columns per 32 grid points
+  double fd = 0;
+  for (int st = 0; st < 4; ++st) {
+    const int nx = (st == 0 || st == 2) ? 4 : 1;
+    for (int p = h->fac_first[st]; p < h->fac_first[st] + h->fac_n[st]; ++p) {
+      const int ix = pairs[p].x, iy = pairs[p].y;
+      const int ny = h->nsu[iy], dy = h->db[iy];
+      const int bt = (ny + 15) / 16 + (dy - ny + 15) / 16;
+      fd += (double)bt * (blk[ix].nc0 + blk[ix].nc1) * nx * 2 * (2.0 * 16 * 16 * FAC_CH);
+    }
+  }
+  h->flops_unit[K_FAC_DENS] = fd * (RP / 16);
+  double fpj = 0;
+  for (int k = 0; k < 16; ++k) {
+    const int nrt = k & 7, kind = k >> 3;
+    fpj += (double)h->pj_count[k] * nrt * (kind ? 1 : 5) * 2 * (2.0 * 16 * 16 * FAC_RT);
+  }
+  h->flops_unit[K_FAC_PROJ] = fpj * (RP / FAC_RT);
+  h->fac_cap = 0;
+  h->fac_ready = true;
+  return 0;
+}
+
+constexpr int FAC_DPART = 32;   // partial density planes (parts x slots) of the split density launch
+
+int fac_workspace(pnfam_gpu_ctx *h, int cap) {
+  if (cap <= h->fac_cap) return 0;
+  const size_t n = (size_t)cap * 2 * h->fac_L * 2;
+  if (int e = h->FRh.alloc(n)) return e;
+  if (int e = h->FRl.alloc(n)) return e;
+  h->fac_cap = cap;
+  return 0;
+}
+
+// calc_hamiltonian for the active columns with the factored split-BF16 kernels.
+int stage_hamiltonian_fac(pnfam_gpu_ctx *h, int na, const int *act, int n3) {
+  if (int e = fac_workspace(h, h->cap)) return e;
+  { Prof pr(h, K_FAC_CVT, na);
+    if (h->fac_nseg)
+      hipLaunchKernelGGL(k_fac_rho, dim3((h->fac_nseg + 3) / 4, na), dim3(256), 0, h->stream, h->d_rsp.as<double>(),
+                         h->d_fac_segs.as<FacSeg>(), h->fac_nseg, h->nxy, h->fac_L, act, h->FRh.as<__bf16>(),
+                         h->FRl.as<__bf16>()); }
+  FacDensArgs D{};
+  D.FRh = h->FRh.as<__bf16>(); D.FRl = h->FRl.as<__bf16>(); D.L = h->fac_L;
+  D.DBh = h->DBh.as<__bf16>(); D.DBl = h->DBl.as<__bf16>(); D.dbx = h->fac_dbx;
+  D.blk = h->d_fac_blk.as<FacBlock>(); D.pairs = h->d_fac_pairs.as<int4>();
+  D.rho_first[0] = h->fac_first[0]; D.rho_n[0] = h->fac_n[0]; D.kap_first[0] = h->fac_first[1]; D.kap_n[0] = h->fac_n[1];
+  D.rho_first[1] = h->fac_first[2]; D.rho_n[1] = h->fac_n[2]; D.kap_first[1] = h->fac_first[3]; D.kap_n[1] = h->fac_n[3];
+  D.wf = h->d_wf.as<double>(); D.wfdr = h->d_wfdr.as<double>(); D.wfdp = h->d_wfdp.as<double>();
+  D.wfdz = h->d_wfdz.as<double>(); D.wdcori = h->d_wdcori.as<double>();
+  D.isstart = h->d_isstart.as<int>(); D.db = h->d_db.as<int>(); D.nsu = h->d_nsu.as<int>();
+  D.nghl = h->nghl; D.RP = h->RP; D.dens = h->d_dens.as<double>(); D.act = act; D.n3 = n3;
+  // few points: the pairs of a grid tile spread over psplit workgroups (psplit * na <= FAC_DPART)
+  D.psplit = std::max(1, std::min(8, FAC_DPART / na));
+  if (D.psplit > 1) {
+    if (int e = h->d_fac_dpart.alloc((size_t)FAC_DPART * 2 * 2 * NDENS * h->nghl * 4)) return e;
+    D.part = h->d_fac_dpart.as<float>();
+  }
+  { Prof pr(h, K_FAC_DENS, na + 2 * n3);
+    hipLaunchKernelGGL(k_fac_density, dim3(h->RP / 16, na, 2 * D.psplit), dim3(64 * FAC_NW), 0, h->stream, D);
+    if (D.psplit > 1)
+      hipLaunchKernelGGL(k_fac_dsum, dim3((2 * NDENS * h->nghl + 255) / 256, na, 2), dim3(256), 0, h->stream, D.part,
+                         D.psplit, na, h->nghl, h->d_wdcori.as<double>(), act, h->d_dens.as<double>()); }
+  if (int e = stage_fields(h, na, act)) return e;
+  { Prof pr(h, K_ZERO, na);
+    hipLaunchKernelGGL(k_zero_slots, dim3(64, na), dim3(256), 0, h->stream, h->d_hsp.as<double>(), act, 8, 0, 8, h->nxy); }
+  FacProjArgs P{};
+  P.PAh = h->PAh.as<__bf16>(); P.PAl = h->PAl.as<__bf16>(); P.pz = h->fac_pz;
+  P.psi[0] = h->d_wf.as<double>(); P.psi[1] = h->d_wfdr.as<double>(); P.psi[2] = h->d_wfdp.as<double>();
+  P.psi[3] = h->d_wfdz.as<double>(); P.psi[4] = h->d_wfd2.as<double>();
+  P.isstart = h->d_isstart.as<int>(); P.db = h->d_db.as<int>(); P.nsu = h->d_nsu.as<int>();
+  P.flds = h->d_flds.as<double>(); P.hsp = h->d_hsp.as<double>(); P.nxy = h->nxy; P.nghl = h->nghl; P.RP = h->RP;
+  P.act = act; P.n3 = n3;
+  { Prof pr(h, K_FAC_PROJ, na + 2 * n3);
+    const Tile2 *tl = h->d_tiles2.as<Tile2>();
+    for (int k = 0; k < 16; ++k) {
+      const int n = h->pj_count[k];
+      if (!n) continue;
+      const dim3 grid(n, na);
+      const Tile2 *tk = tl + h->pj_first[k];
+#define FP_LAUNCH(NRT, KIND) hipLaunchKernelGGL((k_fac_project<NRT, KIND>), grid, dim3(64 * FAC_NW), 0, h->stream, P, tk)
+      switch (k) {
+        case 1: FP_LAUNCH(1, 0); break;
+        case 2: FP_LAUNCH(2, 0); break;
+        case 3: FP_LAUNCH(3, 0); break;
+        case 4: FP_LAUNCH(4, 0); break;
+        case 9: FP_LAUNCH(1, 1); break;
+        case 10: FP_LAUNCH(2, 1); break;
+        case 11: FP_LAUNCH(3, 1); break;
+        case 12: FP_LAUNCH(4, 1); break;
+        default: return fail(PNFAM_GPU_EINVAL, "projection tile class out of range");
+      }
+#undef FP_LAUNCH
+    } }
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+// Grid stage of the mixed path for na active points, the first n3 of which
+// take three BF16 products: the pair-table GEMMs or the factored kernels
+// (PNFAM_GPU_MIX_GRID = tables | factored | auto).  A pass over the tables
+// costs at least one stream of them whatever the column count, the factored
+// kernels cost per point; auto gives the factored kernels any group of at most
+// PNFAM_GPU_MIX_AUTO_N points: the whole pass, or the few three-product points
+// while the one-product majority streams the tables once.
+int stage_hamiltonian_lowp(pnfam_gpu_ctx *h, int na, const int *act, int n3) {
+  if (h->mix_grid == 2) return stage_hamiltonian_fac(h, na, act, n3);
+  if (h->mix_grid == 1) return stage_hamiltonian_mixed(h, na, act, n3);
+  if (na <= h->mix_auto_n) return stage_hamiltonian_fac(h, na, act, n3);
+  if (n3 > 0 && n3 <= h->mix_auto_n) {
+    if (int e = stage_hamiltonian_fac(h, n3, act, n3)) return e;
+    return stage_hamiltonian_mixed(h, na - n3, act + n3, 0);
+  }
+  return stage_hamiltonian_mixed(h, na, act, n3);
 }
 
 int stage_sp2qp(pnfam_gpu_ctx *h, int na, const int *act) {
@@ -2325,7 +2570,7 @@ static int upload_args(pnfam_gpu_ctx *h, DevPtr &dst, const double *in) {
 static int unit_hamiltonian(pnfam_gpu_ctx *h) {
   if (h->unit_precision == PNFAM_GPU_FP64) return stage_hamiltonian(h, 1, h->d_act.as<int>());
   if (int e = mix_prepare(h)) return e;
-  return stage_hamiltonian_mixed(h, 1, h->d_act.as<int>());
+  return stage_hamiltonian_lowp(h, 1, h->d_act.as<int>(), 1);
 }
 
 int pnfam_gpu_set_unit_precision(pnfam_gpu_ctx *h, int32_t precision) {
@@ -2404,9 +2649,7 @@ int pnfam_gpu_solve_batch(pnfam_gpu_ctx *h, int32_t npts, const double *re_w, co
   const double *e2 = h->bminus ? h->d_En.as<double>() : h->d_Ep.as<double>();
   std::vector<double> si(npts, 1.0), S(2 * npts, 0.0), vmax(npts, 1.0);
   std::vector<int> done(npts, 0), refined(npts, 0), act, act64, actlo;
-  // increments below tau of the amplitudes (judged by the last residual) go
-  // through single BF16 products: their rounding error is scaled by the increment
-  double tau = 1e-3;
+  double tau = 1e-3;   // one-product threshold of the mixed path
   if (const char *ev = std::getenv("PNFAM_GPU_MIX_TAU")) tau = std::atof(ev);
   for (int w = 0; w < npts; ++w) { iters[w] = -1; re_s[w] = 0.0; im_s[w] = 0.0; }
   if (mixed && !skip_res)
@@ -2414,6 +2657,11 @@ int pnfam_gpu_solve_batch(pnfam_gpu_ctx *h, int32_t npts, const double *re_w, co
                        (size_t)npts * N, 0.0);
   const int nwg = h->bro_nwg;
   const long nq = 4 * nxy;   // doubles of H20/H02 (re, im) per frequency
+  // PNFAM_GPU_TRACE=1: one line per iteration on stderr (active points, BF16
+  // products of the grid stage, wall time of the iteration)
+  const bool trace = std::getenv("PNFAM_GPU_TRACE") != nullptr;
+  auto t_last = std::chrono::steady_clock::now();
+  int n3_last = 0;
   for (int iter = 0; iter <= o->max_iter; ++iter) {
     act.clear(); act64.clear(); actlo.clear();
     for (int w = 0; w < npts; ++w) {
@@ -2454,17 +2702,23 @@ int pnfam_gpu_solve_batch(pnfam_gpu_ctx *h, int32_t npts, const double *re_w, co
       }
     }
     if (use_h && !actlo.empty()) {
+      // increments below tau of the amplitudes (judged by the last residual)
+      // take one BF16 product: their rounding error is scaled by the increment.
+      // Points that need three go first; n3 counts them.
+      std::stable_partition(actlo.begin(), actlo.end(), [&](int w) {
+        return iter < 3 || si[w] >= tau * std::max(vmax[w], 1e-300);
+      });
+      int n3 = 0;
+      for (int w : actlo) n3 += (iter < 3 || si[w] >= tau * std::max(vmax[w], 1e-300)) ? 1 : 0;
+      n3_last = n3;
       const int nlo = (int)actlo.size();
       int *dlo = h->d_actlo.as<int>();
       HIP_TRY(hipMemcpyAsync(dlo, actlo.data(), nlo * 4, hipMemcpyHostToDevice, st));
       { Prof pr(h, K_MIXACC, nlo);
         hipLaunchKernelGGL(k_delta, dim3(256, nlo), dim3(256), 0, st, h->d_vin.as<double>(), h->d_vprev.as<double>(),
                            h->d_dvin.as<double>(), N, dlo); }
-      double ratio = 0.0;
-      for (int w : actlo) ratio = std::max(ratio, si[w] / std::max(vmax[w], 1e-300));
-      const int nprod = (iter >= 3 && ratio < tau) ? 1 : 3;
       if (int e = stage_qp2sp(h, h->d_dvin.as<double>(), nlo, dlo)) return e;
-      if (int e = (h->basic_kernels ? stage_hamiltonian(h, nlo, dlo) : stage_hamiltonian_mixed(h, nlo, dlo, nprod)))
+      if (int e = (h->basic_kernels ? stage_hamiltonian(h, nlo, dlo) : stage_hamiltonian_lowp(h, nlo, dlo, n3)))
         return e;
       if (int e = stage_sp2qp(h, nlo, dlo)) return e;
       { Prof pr(h, K_MIXACC, nlo);
@@ -2517,6 +2771,13 @@ int pnfam_gpu_solve_batch(pnfam_gpu_ctx *h, int32_t npts, const double *re_w, co
     HIP_TRY(hipStreamSynchronize(st));
     if (h->prof) prof_collect(h);
     for (int w : act) { re_s[w] = S[2 * w]; im_s[w] = S[2 * w + 1]; }
+    if (trace) {
+      const auto t = std::chrono::steady_clock::now();
+      std::fprintf(stderr, "pnfam_gpu iter %4d active %4d fp64 %4d three-product %4d ms %.3f\n", iter, na,
+                   (int)act64.size(), actlo.empty() ? 0 : n3_last,
+                   std::chrono::duration<double, std::milli>(t - t_last).count());
+      t_last = t;
+    }
   }
   if (si_final) for (int w = 0; w < npts; ++w) si_final[w] = si[w];
   if (x_out) HIP_TRY(hipMemcpy(x_out, h->d_vin.p, (size_t)npts * N * 8, hipMemcpyDeviceToHost));

@@ -17,10 +17,12 @@ pytestmark = pytest.mark.gpu
 TOL = {"MIXED": 1e-6, "MIXED_NOREFINE": 1e-4}
 
 
+@pytest.mark.parametrize("grid", ["tables", "factored"])
 @pytest.mark.parametrize("mode", ["MIXED", "MIXED_NOREFINE"])
 @pytest.mark.parametrize("nucleus,K", [("S40", 0), ("S40", 1), ("Gd162", 0), ("Gd162", 1)])
-def test_mixed_matches_reference_runs(nucleus, K, mode):
+def test_mixed_matches_reference_runs(nucleus, K, mode, grid, monkeypatch):
     from pynfam_amd import engine as E
+    monkeypatch.setenv("PNFAM_GPU_MIX_GRID", grid)
     prob = golden_problem(nucleus, K)
     eng = E.GpuEngine(prob)
     pts = golden_points(nucleus, K)
@@ -29,20 +31,25 @@ def test_mixed_matches_reference_runs(nucleus, K, mode):
     r = eng.solve(w, precision=getattr(E, mode))
     assert np.all(r["iterations"] > 0), r["iterations"]
     err = np.abs(r["strength"] - S) / np.abs(S)
-    print(f"{nucleus} K={K} {mode}: max rel err {err.max():.2e}, iterations {r['iterations'].tolist()}")
+    print(f"{nucleus} K={K} {mode} {grid}: max rel err {err.max():.2e}, iterations {r['iterations'].tolist()}")
     assert err.max() < TOL[mode], err.max()
     eng.close()
 
 
-@pytest.mark.parametrize("gemm", ["rocblas", "own"])
+@pytest.mark.parametrize("gemm", ["rocblas", "own", "factored"])
 @pytest.mark.parametrize("K", [0, 1])
 def test_mixed_grid_stage_against_fp64(K, gemm, monkeypatch):
-    """One calc_hamiltonian through the split-BF16 pair-table contractions
-    (rocBLAS, or the hand-written k_mix_gemm) against the FP64 grid stage on the
-    same random densities: every output array within 1e-4 of its largest element."""
+    """One calc_hamiltonian through the split-BF16 kernels (pair-table
+    contractions on rocBLAS or the hand-written k_mix_gemm, or the factored
+    kernels) against the FP64 grid stage on the same random densities: every
+    output array within 1e-4 of its largest element."""
     from oracle.oracle import Oracle
     from pynfam_amd import engine as E
-    monkeypatch.setenv("PNFAM_GPU_MIX_GEMM", gemm)
+    if gemm == "factored":
+        monkeypatch.setenv("PNFAM_GPU_MIX_GRID", "factored")
+    else:
+        monkeypatch.setenv("PNFAM_GPU_MIX_GRID", "tables")
+        monkeypatch.setenv("PNFAM_GPU_MIX_GEMM", gemm)
     prob = golden_problem("S40", K)
     eng = E.GpuEngine(prob)
     orc = Oracle(prob)
