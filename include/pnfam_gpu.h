@@ -65,16 +65,22 @@ typedef struct {
  *   FP64: every step in FP64.
  *   MIXED: the hamiltonian is linear in the amplitudes, so H(v_k) is carried as
  *     an FP64 sum of H applied to the increments v_k - v_{k-1}; each increment goes
- *     through the grid stage on split-BF16 MFMA with FP32 accumulation.  The
- *     residual, its max-norm, Broyden, G(w) and S stay FP64.  When a point first
+ *     through the block transforms and the grid stage on split-BF16 MFMA with
+ *     FP32 accumulation.  The residual, its max-norm, Broyden, G(w) and S stay
+ *     FP64.  When a point first
  *     meets the tolerance, one full FP64 evaluation replaces the accumulated H
  *     and the point iterates on until it meets the tolerance again.
  *   MIXED_NOREFINE: as MIXED without the final FP64 evaluation; S then carries
  *     the low-precision error of the first increments (about 1e-6 relative).
- *   In both mixed modes an increment whose size, judged by the last residual,
- *   is below 1e-3 of the largest amplitude (environment PNFAM_GPU_MIX_TAU)
+ *   In both mixed modes a point whose increment, judged by its last residual,
+ *   is below 1e-3 of its largest amplitude (environment PNFAM_GPU_MIX_TAU)
  *   goes through single BF16 products instead of the three split products:
- *   its rounding error is scaled down by the increment. */
+ *   its rounding error is scaled down by the increment.  The grid stage runs as
+ *   GEMMs against resident pair tables or as factored kernels whose cost is per
+ *   point; by default each pass takes the factored kernels for groups of at
+ *   most PNFAM_GPU_MIX_AUTO_N (6) points (environment PNFAM_GPU_MIX_GRID =
+ *   tables | factored | auto; PNFAM_GPU_MIX_TRIPROD = bf16 | fp64 for the
+ *   block transforms). */
 #define PNFAM_GPU_FP64 0
 #define PNFAM_GPU_MIXED 1
 #define PNFAM_GPU_MIXED_NOREFINE 2
@@ -125,11 +131,12 @@ int pnfam_gpu_calc_hamiltonian(pnfam_gpu_ctx *h, const double *in, double *out);
 int pnfam_gpu_sp2qp(pnfam_gpu_ctx *h, const double *hsp, double *h20_re, double *h20_im, double *h02_re,
                     double *h02_im);
 
-/* Selects the grid stage used by pnfam_gpu_calc_hamiltonian and
- * pnfam_gpu_densities: PNFAM_GPU_FP64 (default) or PNFAM_GPU_MIXED (the
- * split-BF16 pair-table contractions that the mixed solve applies to its
- * increments).  No counterpart in the reference: a parity hook for the mixed
- * grid stage. */
+/* Selects the arithmetic of the per-unit entry points: PNFAM_GPU_FP64
+ * (default) or PNFAM_GPU_MIXED, the split-BF16 kernels that the mixed solve
+ * applies to its increments (grid stage of pnfam_gpu_calc_hamiltonian and
+ * pnfam_gpu_densities in the form PNFAM_GPU_MIX_GRID selects, block transforms
+ * of pnfam_gpu_qp2sp and pnfam_gpu_sp2qp).  No counterpart in the reference: a
+ * parity hook for the mixed path. */
 int pnfam_gpu_set_unit_precision(pnfam_gpu_ctx *h, int32_t precision);
 
 /* Coordinate-space densities of one sector (diagnostic): dens [56][nghl],

@@ -476,3 +476,145 @@ __global__ void __launch_bounds__(64 * FAC_NW, 2) k_fac_project(FacProjArgs P, c
     }
   }
 }
+
+// ---- block triple products in split BF16 (mixed path) -------------------------
+// As k_triprod_mfma (out = sum_terms s op(A) op(B) op(C)) with the resident
+// block-diagonal A and C as BF16 hi/lo copies, every block padded to a pitch P
+// that is a multiple of 32 and zero-filled: RM[p P + q] = M(p, q) and
+// CM[p P + q] = M(q, p), so every MFMA fragment is 8 contiguous elements.  The
+// per-frequency B is FP64 and split in registers; the strip T = s op(A) op(B)
+// is parked in LDS as FP32 and split again for the product with op(C).  Term
+// fields pad0 / pad1 hold the padded offsets of the A and C blocks.
+struct TpMats {
+  const __bf16 *rmh, *rml, *cmh, *cml;   // [mat][padded elements]
+  long mstride;                          // padded elements per mat
+};
+
+__device__ inline void split8(const double (&d)[8], v8bf &h, v8bf &l) {
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    __bf16 a, b;
+    split_bf16((float)d[i], a, b);
+    h[i] = a;
+    l[i] = b;
+  }
+}
+
+template <int NCT>
+__global__ void __launch_bounds__(64) k_triprod_bf16(const Group *__restrict__ groups, const Term *__restrict__ terms,
+                                                     TpMats M, const double *__restrict__ bbase, long bstride,
+                                                     double *__restrict__ obase, long ostride,
+                                                     const int *__restrict__ act, long nxy, int n3) {
+  constexpr int TW = (16 * NCT + 31) / 32 * 32;   // strip width padded to whole K steps
+  constexpr int PT = TW + 4;
+  __shared__ __attribute__((aligned(16))) float sT[16 * PT];
+  const Group g = groups[blockIdx.x];
+  const int strip = blockIdx.z;
+  const int w = act[blockIdx.y];
+  const bool three = (int)blockIdx.y < n3;
+  const int i0 = strip * 16;
+  if (i0 >= g.nrow) return;                       // uniform per wave
+  const int lane = threadIdx.x, fi = lane & 15, fg = lane >> 4;
+  for (int e = lane; e < 16 * PT; e += 64) sT[e] = 0.0f;   // padding columns stay zero
+  const double *B = bbase + (size_t)w * bstride;
+  double *O = obase + (size_t)w * ostride + (size_t)g.out_slot * nxy + g.out_off;
+  const v4f z4 = {0.f, 0.f, 0.f, 0.f};
+  v4f acc[NCT];
+#pragma unroll
+  for (int c = 0; c < NCT; ++c) acc[c] = z4;
+  __syncthreads();
+  for (int t = 0; t < g.count; ++t) {
+    const Term tm = terms[g.first + t];
+    const int nda = tm.nda, ndc = tm.ndc;
+    const int Pa = (nda + 31) & ~31, Pc = (ndc + 31) & ~31;
+    const size_t ma = (size_t)tm.a_id * M.mstride + tm.pad0, mc = (size_t)tm.c_id * M.mstride + tm.pad1;
+    // op(A) rows contiguous in k: CM for a transposed A, RM otherwise
+    const __bf16 *Ah = (tm.ta ? M.cmh : M.rmh) + ma + (size_t)(i0 + fi) * Pa + 8 * fg;
+    const __bf16 *Al = (tm.ta ? M.cml : M.rml) + ma + (size_t)(i0 + fi) * Pa + 8 * fg;
+    // op(C) columns contiguous in l: RM for a transposed C, CM otherwise
+    const __bf16 *Ch = (tm.tc ? M.rmh : M.cmh) + mc + 8 * fg;
+    const __bf16 *Cl = (tm.tc ? M.rml : M.cml) + mc + 8 * fg;
+    const double *Bb = B + (size_t)tm.b_slot * nxy + tm.b_off;
+    const float sgn = (float)tm.s;
+    // phase 1: T = s op(A)[i0:i0+16, :] op(B)
+    v4f T[NCT];
+#pragma unroll
+    for (int c = 0; c < NCT; ++c) T[c] = z4;
+    for (int k0 = 0; k0 < nda; k0 += 32) {
+      const v8bf ah = *(const v8bf *)(Ah + k0), al = *(const v8bf *)(Al + k0);
+#pragma unroll
+      for (int c = 0; c < NCT; ++c) {
+        const int j = c * 16 + fi;
+        const bool jok = j < ndc;
+        const int jc = jok ? j : 0;
+        double bv[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+          const int k = k0 + 8 * fg + i;
+          const bool ok = jok && k < nda;
+          const int kc = k < nda ? k : 0;
+          const double v = tm.tb ? Bb[jc + (size_t)kc * tm.ldb] : Bb[kc + (size_t)jc * tm.ldb];
+          bv[i] = ok ? v : 0.0;
+        }
+        v8bf bh, bl;
+        split8(bv, bh, bl);
+        T[c] = mfma_split(ah, al, bh, bl, T[c], three);
+      }
+    }
+    // park s T: D[4 fg + v][fi] of column tile c
+#pragma unroll
+    for (int c = 0; c < NCT; ++c)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) sT[(4 * fg + v) * PT + c * 16 + fi] = sgn * T[c][v];
+    __syncthreads();
+    // phase 2: acc += T op(C)
+    for (int l0 = 0; l0 < ndc; l0 += 32) {
+      const float4 t0 = *(const float4 *)&sT[fi * PT + l0 + 8 * fg];
+      const float4 t1 = *(const float4 *)&sT[fi * PT + l0 + 8 * fg + 4];
+      const float tv[8] = {t0.x, t0.y, t0.z, t0.w, t1.x, t1.y, t1.z, t1.w};
+      v8bf th, tl;
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        __bf16 a, b;
+        split_bf16(tv[i], a, b);
+        th[i] = a;
+        tl[i] = b;
+      }
+#pragma unroll
+      for (int c = 0; c < NCT; ++c) {
+        const size_t co = (size_t)(c * 16 + fi) * Pc + l0;
+        const v8bf ch = *(const v8bf *)(Ch + co), cl = *(const v8bf *)(Cl + co);
+        acc[c] = mfma_split(th, tl, ch, cl, acc[c], three);
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int c = 0; c < NCT; ++c) {
+    const int j = c * 16 + fi;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int i = i0 + 4 * fg + v;
+      if (i < g.nrow && j < g.ncol) O[i + (size_t)j * g.nrow] = (double)acc[c][v];
+    }
+  }
+}
+
+// Padded BF16 hi/lo copies (row- and column-major) of one resident block-diagonal
+// matrix: block b of dimension d at FP64 offset off and padded offset poff.
+__global__ void k_tp_copy(const double *__restrict__ m, const int4 *__restrict__ blocks, int nblk,
+                          __bf16 *__restrict__ rmh, __bf16 *__restrict__ rml, __bf16 *__restrict__ cmh,
+                          __bf16 *__restrict__ cml) {
+  const int4 bk = blocks[blockIdx.y];     // (fp64 offset, padded offset, d, P)
+  const int d = bk.z, P = bk.w;
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < P * P; e += gridDim.x * blockDim.x) {
+    const int p = e / P, q = e % P;
+    const float vr = (p < d && q < d) ? (float)m[bk.x + p + (size_t)q * d] : 0.0f;   // M(p, q)
+    const float vc = (p < d && q < d) ? (float)m[bk.x + q + (size_t)p * d] : 0.0f;   // M(q, p)
+    __bf16 h, l;
+    split_bf16(vr, h, l);
+    rmh[bk.y + e] = h; rml[bk.y + e] = l;
+    split_bf16(vc, h, l);
+    cmh[bk.y + e] = h; cml[bk.y + e] = l;
+  }
+}

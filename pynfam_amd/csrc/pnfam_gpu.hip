@@ -1351,6 +1351,12 @@ struct pnfam_gpu_ctx {
   long fac_L = 0, fac_pz = 0, fac_dbx = 0;
   int fac_nseg = 0, fac_cap = 0;
   int fac_first[4] = {0, 0, 0, 0}, fac_n[4] = {0, 0, 0, 0};   // rho0, kap0, rho1, kap1 pair ranges
+  // split-BF16 block triple products (mixed path): padded hi/lo copies of U/V
+  bool tp_bf16 = true;          // PNFAM_GPU_MIX_TRIPROD = bf16 | fp64
+  bool tp_ready = false;
+  std::vector<int4> tp_blocks;  // (fp64 offset, padded offset, d, P) per block
+  long tp_mstride = 0;
+  DevPtr tp_rmh, tp_rml, tp_cmh, tp_cml, d_tp_blocks;
   int unit_precision = PNFAM_GPU_FP64;   // grid stage of the per-unit entry points
   int bro_nwg = 0;
   double flops_q2s = 0, flops_s2q = 0;
@@ -1473,6 +1479,87 @@ int launch_triprod(pnfam_gpu_ctx *h, int stage, int ri, const Group *groups, con
   return 0;
 }
 
+// Padded layout of the resident block-diagonal matrices for the BF16 triple
+// products: block i (dimension d, FP64 offset sum d_j^2) at the padded offset
+// sum P_j^2, P = d rounded up to 32.
+void tp_layout(pnfam_gpu_ctx *h) {
+  h->tp_blocks.clear();
+  long off = 0, poff = 0;
+  for (size_t i = 0; i < h->db.size(); ++i) {
+    const int d = h->db[i], P = (d + 31) & ~31;
+    h->tp_blocks.push_back(make_int4((int)off, (int)poff, d, P));
+    off += (long)d * d;
+    poff += (long)P * P;
+  }
+  h->tp_mstride = std::max<long>(poff, 8);
+}
+
+// Term fields pad0 / pad1: padded offsets of the A and C blocks (-1 if the FP64
+// offset is not a block start).
+void tp_patch_terms(const pnfam_gpu_ctx *h, std::vector<Term> &terms) {
+  auto pad = [&](int off) {
+    for (const auto &b : h->tp_blocks)
+      if (b.x == off) return b.y;
+    return -1;
+  };
+  for (auto &t : terms) { t.pad0 = pad(t.a_off); t.pad1 = pad(t.c_off); }
+}
+
+bool tp_terms_ok(const std::vector<Term> &terms) {
+  for (const auto &t : terms)
+    if (t.pad0 < 0 || t.pad1 < 0) return false;
+  return true;
+}
+
+int tp_prepare(pnfam_gpu_ctx *h) {
+  if (h->tp_ready) return 0;
+  const size_t n = 4 * (size_t)h->tp_mstride * 2;
+  int e;
+  if ((e = h->tp_rmh.alloc(n)) || (e = h->tp_rml.alloc(n)) || (e = h->tp_cmh.alloc(n)) || (e = h->tp_cml.alloc(n)) ||
+      (e = upload(h->d_tp_blocks, h->tp_blocks.data(), h->tp_blocks.size() * sizeof(int4), h->stream)))
+    return e;
+  for (int m = 0; m < 4; ++m) {
+    const size_t o = (size_t)m * h->tp_mstride;
+    hipLaunchKernelGGL(k_tp_copy, dim3(16, (unsigned)h->tp_blocks.size()), dim3(256), 0, h->stream,
+                       h->d_U[m].as<double>(), h->d_tp_blocks.as<int4>(), (int)h->tp_blocks.size(),
+                       h->tp_rmh.as<__bf16>() + o, h->tp_rml.as<__bf16>() + o, h->tp_cmh.as<__bf16>() + o,
+                       h->tp_cml.as<__bf16>() + o);
+  }
+  HIP_TRY(hipGetLastError());
+  h->tp_ready = true;
+  return 0;
+}
+
+int launch_triprod_bf16(pnfam_gpu_ctx *h, int stage, int ri, const Group *groups, const Term *terms,
+                        const double *bbase, long bstride, double *obase, long ostride, int na, const int *act,
+                        int n3) {
+  if (int e = tp_prepare(h)) return e;
+  TpMats M{h->tp_rmh.as<__bf16>(), h->tp_rml.as<__bf16>(), h->tp_cmh.as<__bf16>(), h->tp_cml.as<__bf16>(),
+           h->tp_mstride};
+  for (int c = 1; c <= TP_MAXCT; ++c) {
+    const int n = h->tp_count[stage][ri][c];
+    if (!n) continue;
+    const dim3 grid(n, na, h->tp_strips[stage][ri][c]);
+    const Group *g = groups + h->tp_first[stage][ri][c];
+#define TB_LAUNCH(NC)                                                                                          \
+  hipLaunchKernelGGL(k_triprod_bf16<NC>, grid, dim3(64), 0, h->stream, g, terms, M, bbase, bstride, obase, ostride, \
+                     act, h->nxy, n3)
+    switch (c) {
+      case 1: TB_LAUNCH(1); break;
+      case 2: TB_LAUNCH(2); break;
+      case 3: TB_LAUNCH(3); break;
+      case 4: TB_LAUNCH(4); break;
+      case 5: TB_LAUNCH(5); break;
+      case 6: TB_LAUNCH(6); break;
+      case 7: TB_LAUNCH(7); break;
+      default: TB_LAUNCH(8); break;
+    }
+#undef TB_LAUNCH
+  }
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
 int ensure_workspace(pnfam_gpu_ctx *h, int npts, int M) {
   if (npts <= h->cap && M == h->hist_cap) return 0;
   const size_t n = (size_t)std::max(npts, 1);
@@ -1516,13 +1603,21 @@ dim3 grid1(size_t n, int bs, int maxb = 2048) {
 }
 
 // ---- pipeline stages over the active list (device pointer act, count na) --
-int stage_qp2sp(pnfam_gpu_ctx *h, const double *vin, int na, const int *act) {
+// lowp: split-BF16 triple products (the mixed path's increments), the first
+// n3 active points with three products.
+int stage_qp2sp(pnfam_gpu_ctx *h, const double *vin, int na, const int *act, bool lowp = false, int n3 = 0) {
   const long nxy = h->nxy;
   { Prof pr(h, K_ZERO, na);
     hipLaunchKernelGGL(k_zero_slots, dim3(64, na), dim3(256), 0, h->stream, h->d_rsp.as<double>(), act, 8, 0, 8, nxy); }
   Prof pr(h, K_TRIPROD_Q2S, na);
   for (int ri = 0; ri < 2; ++ri) {
     if (!h->ngroups_q2s[ri]) continue;
+    if (lowp) {
+      if (int e = launch_triprod_bf16(h, 0, ri, h->d_groups_q2s[ri].as<Group>(), h->d_terms_q2s[ri].as<Term>(), vin,
+                                      4 * nxy, h->d_rsp.as<double>(), 8 * nxy, na, act, n3))
+        return e;
+      continue;
+    }
     if (!h->basic_kernels) {
       if (int e = launch_triprod(h, 0, ri, h->d_groups_q2s[ri].as<Group>(), h->d_terms_q2s[ri].as<Term>(), vin, 4 * nxy,
                                  h->d_rsp.as<double>(), 8 * nxy, na, act))
@@ -2147,13 +2242,19 @@ int stage_hamiltonian_lowp(pnfam_gpu_ctx *h, int na, const int *act, int n3) {
   return stage_hamiltonian_mixed(h, na, act, n3);
 }
 
-int stage_sp2qp(pnfam_gpu_ctx *h, int na, const int *act) {
+int stage_sp2qp(pnfam_gpu_ctx *h, int na, const int *act, bool lowp = false, int n3 = 0) {
   const long nxy = h->nxy;
   { Prof pr(h, K_ZERO, na);
     hipLaunchKernelGGL(k_zero_slots, dim3(64, na), dim3(256), 0, h->stream, h->d_hqp.as<double>(), act, 4, 0, 4, nxy); }
   Prof pr(h, K_TRIPROD_S2Q, na);
   for (int ri = 0; ri < 2; ++ri) {
     if (!h->ngroups_s2q[ri]) continue;
+    if (lowp) {
+      if (int e = launch_triprod_bf16(h, 1, ri, h->d_groups_s2q[ri].as<Group>(), h->d_terms_s2q[ri].as<Term>(),
+                                      h->d_hsp.as<double>(), 8 * nxy, h->d_hqp.as<double>(), 4 * nxy, na, act, n3))
+        return e;
+      continue;
+    }
     if (!h->basic_kernels) {
       if (int e = launch_triprod(h, 1, ri, h->d_groups_s2q[ri].as<Group>(), h->d_terms_s2q[ri].as<Term>(),
                                  h->d_hsp.as<double>(), 8 * nxy, h->d_hqp.as<double>(), 4 * nxy, na, act))
@@ -2207,6 +2308,11 @@ int pnfam_gpu_create(const pnfam_gpu_basis *b, pnfam_gpu_ctx **out) {
         return fail(PNFAM_GPU_EINVAL, "states must be spin-sorted within blocks");
   h->cscal.assign(b->cscal, b->cscal + 12);
   if (const char *ev = std::getenv("PNFAM_GPU_BASIC")) h->basic_kernels = ev[0] == '1';
+  if (const char *ev = std::getenv("PNFAM_GPU_MIX_TRIPROD")) {
+    if (std::strcmp(ev, "fp64") == 0) h->tp_bf16 = false;
+    else if (std::strcmp(ev, "bf16") != 0) return fail(PNFAM_GPU_EINVAL, "PNFAM_GPU_MIX_TRIPROD must be bf16 or fp64");
+  }
+  tp_layout(h.get());
   HIP_TRY(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
   const size_t tab = (size_t)b->nghl * b->dqp * 8;
   size_t dmat = 0;
@@ -2309,6 +2415,8 @@ int pnfam_gpu_set_operator(pnfam_gpu_ctx *h, int32_t beta_minus, const int32_t *
     if (ri == 0) h->flops_q2s = 0;
     for (auto &t : terms) h->flops_q2s += 2.0 * t.nda * t.nda * t.ndc + 2.0 * t.nda * t.ndc * t.ndc;
     sort_groups_by_tiles(h, 0, ri, groups);
+    tp_patch_terms(h, terms);
+    if (!tp_terms_ok(terms)) h->tp_bf16 = false;   // A or C not whole blocks: FP64 products only
     int e;
     if ((e = upload(h->d_terms_q2s[ri], terms.data(), std::max<size_t>(1, terms.size()) * sizeof(Term), h->stream)) ||
         (e = upload(h->d_groups_q2s[ri], groups.data(), std::max<size_t>(1, groups.size()) * sizeof(Group), h->stream)))
@@ -2344,6 +2452,8 @@ int pnfam_gpu_set_operator(pnfam_gpu_ctx *h, int32_t beta_minus, const int32_t *
     if (ri == 0) h->flops_s2q = 0;
     for (auto &t : terms) h->flops_s2q += 2.0 * t.nda * t.nda * t.ndc + 2.0 * t.nda * t.ndc * t.ndc;
     sort_groups_by_tiles(h, 1, ri, groups);
+    tp_patch_terms(h, terms);
+    if (!tp_terms_ok(terms)) h->tp_bf16 = false;   // A or C not whole blocks: FP64 products only
     int e;
     if ((e = upload(h->d_terms_s2q[ri], terms.data(), std::max<size_t>(1, terms.size()) * sizeof(Term), h->stream)) ||
         (e = upload(h->d_groups_s2q[ri], groups.data(), std::max<size_t>(1, groups.size()) * sizeof(Group), h->stream)))
@@ -2549,7 +2659,7 @@ int pnfam_gpu_qp2sp(pnfam_gpu_ctx *h, const double *x_re, const double *x_im, co
   HIP_TRY(hipMemcpyAsync(v + n, y_re, n * 8, hipMemcpyHostToDevice, h->stream));
   HIP_TRY(hipMemcpyAsync(v + 2 * n, x_im, n * 8, hipMemcpyHostToDevice, h->stream));
   HIP_TRY(hipMemcpyAsync(v + 3 * n, y_im, n * 8, hipMemcpyHostToDevice, h->stream));
-  if (int e = stage_qp2sp(h, v, 1, h->d_act.as<int>())) return e;
+  if (int e = stage_qp2sp(h, v, 1, h->d_act.as<int>(), h->unit_precision != PNFAM_GPU_FP64 && h->tp_bf16, 1)) return e;
   HIP_TRY(hipMemcpyAsync(rsp, h->d_rsp.p, 8 * n * 8, hipMemcpyDeviceToHost, h->stream));
   HIP_TRY(hipStreamSynchronize(h->stream));
   return 0;
@@ -2609,7 +2719,7 @@ int pnfam_gpu_sp2qp(pnfam_gpu_ctx *h, const double *hsp, double *h20_re, double 
   if (int e = one_point(h)) return e;
   const long n = h->nxy;
   if (int e = upload_args(h, h->d_hsp, hsp)) return e;
-  if (int e = stage_sp2qp(h, 1, h->d_act.as<int>())) return e;
+  if (int e = stage_sp2qp(h, 1, h->d_act.as<int>(), h->unit_precision != PNFAM_GPU_FP64 && h->tp_bf16, 1)) return e;
   const double *q = h->d_hqp.as<double>();
   HIP_TRY(hipMemcpyAsync(h20_re, q, n * 8, hipMemcpyDeviceToHost, h->stream));
   HIP_TRY(hipMemcpyAsync(h02_re, q + n, n * 8, hipMemcpyDeviceToHost, h->stream));
@@ -2717,10 +2827,11 @@ int pnfam_gpu_solve_batch(pnfam_gpu_ctx *h, int32_t npts, const double *re_w, co
       { Prof pr(h, K_MIXACC, nlo);
         hipLaunchKernelGGL(k_delta, dim3(256, nlo), dim3(256), 0, st, h->d_vin.as<double>(), h->d_vprev.as<double>(),
                            h->d_dvin.as<double>(), N, dlo); }
-      if (int e = stage_qp2sp(h, h->d_dvin.as<double>(), nlo, dlo)) return e;
+      const bool tpl = h->tp_bf16 && !h->basic_kernels;
+      if (int e = stage_qp2sp(h, h->d_dvin.as<double>(), nlo, dlo, tpl, n3)) return e;
       if (int e = (h->basic_kernels ? stage_hamiltonian(h, nlo, dlo) : stage_hamiltonian_lowp(h, nlo, dlo, n3)))
         return e;
-      if (int e = stage_sp2qp(h, nlo, dlo)) return e;
+      if (int e = stage_sp2qp(h, nlo, dlo, tpl, n3)) return e;
       { Prof pr(h, K_MIXACC, nlo);
         hipLaunchKernelGGL(k_hacc, dim3(256, nlo), dim3(256), 0, st, h->d_hacc.as<double>(), h->d_hqp.as<double>(),
                            nq, iter == 1 ? 1 : 0, dlo); }

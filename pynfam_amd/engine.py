@@ -181,7 +181,9 @@ class GpuEngine:
 
     # ---- per-unit entry points ---------------------------------------------
     def set_unit_precision(self, precision):
-        """Grid stage used by calc_hamiltonian() and densities(): FP64 or MIXED."""
+        """Arithmetic of the per-unit entry points: FP64, or MIXED (the split-BF16
+        kernels of the mixed path: grid stage of calc_hamiltonian() and
+        densities(), block transforms of qp2sp() and sp2qp())."""
         self._check(self.lib.pnfam_gpu_set_unit_precision(self.h, int(precision)))
 
     def structures(self):

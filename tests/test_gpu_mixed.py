@@ -76,6 +76,35 @@ def test_mixed_grid_stage_against_fp64(K, gemm, monkeypatch):
     eng.close()
 
 
+@pytest.mark.parametrize("K", [0, 1])
+def test_mixed_transforms_against_fp64(K):
+    """The split-BF16 block transforms (qp -> sp on random amplitudes, sp -> qp
+    on random single-particle fields, both S40 quadrant structures) against the
+    FP64 ones: every output array within 1e-4 of its largest element."""
+    from pynfam_amd import engine as E
+    prob = golden_problem("S40", K)
+    eng = E.GpuEngine(prob)
+    rng = np.random.default_rng(400 + K)
+    amp = [rng.standard_normal(prob.nxy) for _ in range(4)]
+    hsp = rng.standard_normal((8, prob.nxy))
+    ref_sp = eng.qp2sp(*amp)
+    ref_qp = eng.sp2qp(hsp)
+    eng.set_unit_precision(E.MIXED)
+    got_sp = eng.qp2sp(*amp)
+    got_qp = eng.sp2qp(hsp)
+    eng.set_unit_precision(E.FP64)
+    for name, ref, got in (("qp2sp", ref_sp, got_sp), ("sp2qp", ref_qp, got_qp)):
+        for k in range(len(ref)):
+            scale = np.max(np.abs(ref[k]))
+            if scale == 0.0:
+                assert np.max(np.abs(got[k])) == 0.0, (name, k)
+                continue
+            err = np.max(np.abs(got[k] - ref[k])) / scale
+            print(f"K={K} {name} array {k}: max err / max {err:.2e}")
+            assert err < 1e-4, (name, k, err)
+    eng.close()
+
+
 def test_mixed_edge_cases():
     """Interrupted points, no residual interaction and Broyden history 0 behave
     as on the FP64 path."""
