@@ -490,20 +490,17 @@ struct TpMats {
   long mstride;                          // padded elements per mat
 };
 
-__device__ inline void split8(const double (&d)[8], v8bf &h, v8bf &l) {
-#pragma unroll
-  for (int i = 0; i < 8; ++i) {
-    __bf16 a, b;
-    split_bf16((float)d[i], a, b);
-    h[i] = a;
-    l[i] = b;
-  }
-}
+// B operands converted once per pass (k_tp_bconv): per active point and term,
+// op(B) as [j][k] (16 NCT x Pa, zero-padded), hi and lo.
+struct TpB {
+  const __bf16 *h, *l;
+  long L;                                // elements per active point
+  const int *toff;                       // per term
+};
 
 template <int NCT>
 __global__ void __launch_bounds__(64) k_triprod_bf16(const Group *__restrict__ groups, const Term *__restrict__ terms,
-                                                     TpMats M, const double *__restrict__ bbase, long bstride,
-                                                     double *__restrict__ obase, long ostride,
+                                                     TpMats M, TpB BB, double *__restrict__ obase, long ostride,
                                                      const int *__restrict__ act, long nxy, int n3) {
   constexpr int TW = (16 * NCT + 31) / 32 * 32;   // strip width padded to whole K steps
   constexpr int PT = TW + 4;
@@ -516,7 +513,6 @@ __global__ void __launch_bounds__(64) k_triprod_bf16(const Group *__restrict__ g
   if (i0 >= g.nrow) return;                       // uniform per wave
   const int lane = threadIdx.x, fi = lane & 15, fg = lane >> 4;
   for (int e = lane; e < 16 * PT; e += 64) sT[e] = 0.0f;   // padding columns stay zero
-  const double *B = bbase + (size_t)w * bstride;
   double *O = obase + (size_t)w * ostride + (size_t)g.out_slot * nxy + g.out_off;
   const v4f z4 = {0.f, 0.f, 0.f, 0.f};
   v4f acc[NCT];
@@ -534,8 +530,12 @@ __global__ void __launch_bounds__(64) k_triprod_bf16(const Group *__restrict__ g
     // op(C) columns contiguous in l: RM for a transposed C, CM otherwise
     const __bf16 *Ch = (tm.tc ? M.rmh : M.cmh) + mc + 8 * fg;
     const __bf16 *Cl = (tm.tc ? M.rml : M.cml) + mc + 8 * fg;
-    const double *Bb = B + (size_t)tm.b_slot * nxy + tm.b_off;
+    const size_t bo = (size_t)blockIdx.y * BB.L + BB.toff[g.first + t] + (size_t)fi * Pa + 8 * fg;
+    const __bf16 *Bh = BB.h + bo, *Bl = BB.l + bo;
     const float sgn = (float)tm.s;
+    // column tiles of this term (a launch over several tile classes uses the
+    // widest template; tiles beyond the term's own are skipped, wave-uniformly)
+    const int nct = (ndc + 15) / 16;
     // phase 1: T = s op(A)[i0:i0+16, :] op(B)
     v4f T[NCT];
 #pragma unroll
@@ -544,20 +544,9 @@ __global__ void __launch_bounds__(64) k_triprod_bf16(const Group *__restrict__ g
       const v8bf ah = *(const v8bf *)(Ah + k0), al = *(const v8bf *)(Al + k0);
 #pragma unroll
       for (int c = 0; c < NCT; ++c) {
-        const int j = c * 16 + fi;
-        const bool jok = j < ndc;
-        const int jc = jok ? j : 0;
-        double bv[8];
-#pragma unroll
-        for (int i = 0; i < 8; ++i) {
-          const int k = k0 + 8 * fg + i;
-          const bool ok = jok && k < nda;
-          const int kc = k < nda ? k : 0;
-          const double v = tm.tb ? Bb[jc + (size_t)kc * tm.ldb] : Bb[kc + (size_t)jc * tm.ldb];
-          bv[i] = ok ? v : 0.0;
-        }
-        v8bf bh, bl;
-        split8(bv, bh, bl);
+        if (c >= nct) break;
+        const size_t o = (size_t)c * 16 * Pa + k0;
+        const v8bf bh = *(const v8bf *)(Bh + o), bl = *(const v8bf *)(Bl + o);
         T[c] = mfma_split(ah, al, bh, bl, T[c], three);
       }
     }
@@ -582,6 +571,7 @@ __global__ void __launch_bounds__(64) k_triprod_bf16(const Group *__restrict__ g
       }
 #pragma unroll
       for (int c = 0; c < NCT; ++c) {
+        if (c >= nct) break;
         const size_t co = (size_t)(c * 16 + fi) * Pc + l0;
         const v8bf ch = *(const v8bf *)(Ch + co), cl = *(const v8bf *)(Cl + co);
         acc[c] = mfma_split(th, tl, ch, cl, acc[c], three);
@@ -616,5 +606,33 @@ __global__ void k_tp_copy(const double *__restrict__ m, const int4 *__restrict__
     rmh[bk.y + e] = h; rml[bk.y + e] = l;
     split_bf16(vc, h, l);
     cmh[bk.y + e] = h; cml[bk.y + e] = l;
+  }
+}
+
+// op(B) of every term into the padded [j][k] BF16 hi/lo layout of k_triprod_bf16:
+// one job = up to TP_BJOB elements of one term; blockIdx.y = active point.
+constexpr int TP_BJOB = 2048;
+__global__ void __launch_bounds__(256) k_tp_bconv(const Term *__restrict__ terms, const int *__restrict__ toff,
+                                                  const int2 *__restrict__ jobs, const double *__restrict__ bbase,
+                                                  long bstride, long nxy, const int *__restrict__ act, long L,
+                                                  __bf16 *__restrict__ oh, __bf16 *__restrict__ ol) {
+  const int2 jb = jobs[blockIdx.x];          // (term, first element)
+  const int slot = blockIdx.y;
+  const int w = act[slot];
+  const Term tm = terms[jb.x];
+  const int nda = tm.nda, ndc = tm.ndc;
+  const int Pa = (nda + 31) & ~31;
+  const int n = (ndc + 15) / 16 * 16 * Pa;
+  const double *Bb = bbase + (size_t)w * bstride + (size_t)tm.b_slot * nxy + tm.b_off;
+  const size_t o = (size_t)slot * L + toff[jb.x];
+  const int e1 = min(n, jb.y + TP_BJOB);
+  for (int e = jb.y + threadIdx.x; e < e1; e += blockDim.x) {
+    const int j = e / Pa, k = e - (e / Pa) * Pa;
+    float v = 0.0f;
+    if (j < ndc && k < nda) v = (float)(tm.tb ? Bb[j + (size_t)k * tm.ldb] : Bb[k + (size_t)j * tm.ldb]);
+    __bf16 h, l;
+    split_bf16(v, h, l);
+    oh[o + e] = h;
+    ol[o + e] = l;
   }
 }

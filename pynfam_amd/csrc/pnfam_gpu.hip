@@ -146,6 +146,7 @@ __global__ void __launch_bounds__(256) k_triprod(const Group *__restrict__ group
 }
 
 constexpr int TP_MAXCT = 8;    // column tiles of 16 per strip (block sizes up to 128)
+constexpr int TP_MERGE_N = 8;  // at most this many points: one triple-product launch over all tile classes
 
 // Block triple products on the matrix cores.  One wave per (group, 16-row
 // strip of the output block, frequency): T_strip = s op(A)_strip op(B) is
@@ -1113,16 +1114,23 @@ __global__ void __launch_bounds__(BRO_WG) k_bro_resid(BroArgs B) {
   }
 }
 
-__global__ void k_bro_norm(BroArgs B) {
+// One wave per frequency: lane l folds the partials l, l + 64, ... in order,
+// then the lanes are combined by a fixed butterfly.
+__global__ void __launch_bounds__(64) k_bro_norm(BroArgs B) {
   const int w = B.act[blockIdx.x];
-  if (threadIdx.x != 0) return;
   double m = 0.0, s = 0.0, v = 0.0;
-  for (int k = 0; k < B.nwg; ++k) {
+  for (int k = threadIdx.x; k < B.nwg; k += 64) {
     const double *pp = B.part + ((size_t)w * B.nwg + k) * (2 * BRO_MAXM + 2);
     m = fmax(m, pp[0]);
     s += pp[1];
     v = fmax(v, pp[2]);
   }
+  for (int o = 32; o > 0; o >>= 1) {
+    m = fmax(m, __shfl_xor(m, o, 64));
+    s += __shfl_xor(s, o, 64);
+    v = fmax(v, __shfl_xor(v, o, 64));
+  }
+  if (threadIdx.x != 0) return;
   B.si[w] = m;
   B.vmax[w] = v;
   B.scal[(size_t)w * 4 + 0] = (B.iter >= 2 && B.M > 0) ? 1.0 / sqrt(s) : 1.0;
@@ -1179,16 +1187,26 @@ __global__ void __launch_bounds__(256) k_bro_solve(BroArgs B) {
   const int w = B.act[blockIdx.x];
   const int M = B.M, k = B.iter_used;
   double *gram = B.gram + (size_t)w * M * M;
-  // reduce partial dots
-  for (int j = threadIdx.x; j < k; j += blockDim.x) {
-    double sg = 0.0, sv = 0.0;
-    for (int q = 0; q < B.nwg; ++q) {
-      const double *pp = B.part + ((size_t)w * B.nwg + q) * (2 * BRO_MAXM + 2) + 2;
-      sg += pp[2 * j];
-      sv += pp[2 * j + 1];
+  // reduce the partial dots: one wave per history entry j, lane l folds the
+  // partials l, l + 64, ... in order, then a fixed butterfly over the lanes
+  {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
+    for (int j = wv; j < k; j += nwv) {
+      double sg = 0.0, sv = 0.0;
+      for (int q = lane; q < B.nwg; q += 64) {
+        const double *pp = B.part + ((size_t)w * B.nwg + q) * (2 * BRO_MAXM + 2) + 2;
+        sg += pp[2 * j];
+        sv += pp[2 * j + 1];
+      }
+      for (int o = 32; o > 0; o >>= 1) {
+        sg += __shfl_xor(sg, o, 64);
+        sv += __shfl_xor(sv, o, 64);
+      }
+      if (lane == 0) {
+        if (B.iter >= 2) { gram[(size_t)B.ipos * M + j] = sg; gram[(size_t)j * M + B.ipos] = sg; }
+        work[j] = sv;
+      }
     }
-    if (B.iter >= 2) { gram[(size_t)B.ipos * M + j] = sg; gram[(size_t)j * M + B.ipos] = sg; }
-    work[j] = sv;
   }
   __syncthreads();
   const double w0 = 0.010;
@@ -1357,6 +1375,9 @@ struct pnfam_gpu_ctx {
   std::vector<int4> tp_blocks;  // (fp64 offset, padded offset, d, P) per block
   long tp_mstride = 0;
   DevPtr tp_rmh, tp_rml, tp_cmh, tp_cml, d_tp_blocks;
+  DevPtr d_tp_toff[2][2], d_tp_jobs[2][2], tp_Bh, tp_Bl;   // [stage][ri]: B operand layout and conversion jobs
+  int tp_njobs[2][2] = {{0, 0}, {0, 0}};
+  long tp_L[2] = {0, 0}, tp_Lcap = 0;                        // per active point, per stage
   int unit_precision = PNFAM_GPU_FP64;   // grid stage of the per-unit entry points
   int bro_nwg = 0;
   double flops_q2s = 0, flops_s2q = 0;
@@ -1455,11 +1476,15 @@ int launch_triprod(pnfam_gpu_ctx *h, int stage, int ri, const Group *groups, con
                    long bstride, double *obase, long ostride, int na, const int *act) {
   const double *m0 = h->d_U[0].as<double>(), *m1 = h->d_U[1].as<double>(), *m2 = h->d_U[2].as<double>(),
                *m3 = h->d_U[3].as<double>();
-  for (int c = 1; c <= TP_MAXCT; ++c) {
-    const int n = h->tp_count[stage][ri][c];
+  const bool merge = na <= TP_MERGE_N;
+  int cmax = 0, smax = 0, ntot = 0;
+  for (int c = 1; c <= TP_MAXCT; ++c)
+    if (h->tp_count[stage][ri][c]) { cmax = c; smax = std::max(smax, h->tp_strips[stage][ri][c]); ntot += h->tp_count[stage][ri][c]; }
+  for (int c = merge ? cmax : 1; c <= TP_MAXCT && c >= 1; ++c) {
+    const int n = merge ? ntot : h->tp_count[stage][ri][c];
     if (!n) continue;
-    const dim3 grid(n, na, h->tp_strips[stage][ri][c]);
-    const Group *g = groups + h->tp_first[stage][ri][c];
+    const dim3 grid(n, na, merge ? smax : h->tp_strips[stage][ri][c]);
+    const Group *g = groups + (merge ? 0 : h->tp_first[stage][ri][c]);
 #define TP_LAUNCH(NC)                                                                                      \
   hipLaunchKernelGGL(k_triprod_mfma<NC>, grid, dim3(64), 0, h->stream, g, terms, m0, m1, m2, m3, bbase, bstride, \
                      obase, ostride, act, h->nxy)
@@ -1474,6 +1499,7 @@ int launch_triprod(pnfam_gpu_ctx *h, int stage, int ri, const Group *groups, con
       default: TP_LAUNCH(8); break;
     }
 #undef TP_LAUNCH
+    if (merge) break;
   }
   HIP_TRY(hipGetLastError());
   return 0;
@@ -1505,6 +1531,26 @@ void tp_patch_terms(const pnfam_gpu_ctx *h, std::vector<Term> &terms) {
   for (auto &t : terms) { t.pad0 = pad(t.a_off); t.pad1 = pad(t.c_off); }
 }
 
+// B operand layout of one (stage, ri): term offsets (after base) and
+// conversion jobs; returns the end offset.
+long tp_b_layout(pnfam_gpu_ctx *h, int stage, int ri, const std::vector<Term> &terms, long base, int &err) {
+  std::vector<int> toff(std::max<size_t>(1, terms.size()), 0);
+  std::vector<int2> jobs;
+  long off = base;
+  for (size_t i = 0; i < terms.size(); ++i) {
+    const Term &t = terms[i];
+    const long n = (long)((t.ndc + 15) / 16 * 16) * ((t.nda + 31) & ~31);
+    toff[i] = (int)off;
+    for (long e = 0; e < n; e += TP_BJOB) jobs.push_back(make_int2((int)i, (int)e));
+    off += n;
+  }
+  if (jobs.empty()) jobs.push_back(make_int2(0, 0));
+  h->tp_njobs[stage][ri] = terms.empty() ? 0 : (int)jobs.size();
+  err = upload(h->d_tp_toff[stage][ri], toff.data(), toff.size() * 4, h->stream);
+  if (!err) err = upload(h->d_tp_jobs[stage][ri], jobs.data(), jobs.size() * sizeof(int2), h->stream);
+  return off;
+}
+
 bool tp_terms_ok(const std::vector<Term> &terms) {
   for (const auto &t : terms)
     if (t.pad0 < 0 || t.pad1 < 0) return false;
@@ -1534,16 +1580,33 @@ int launch_triprod_bf16(pnfam_gpu_ctx *h, int stage, int ri, const Group *groups
                         const double *bbase, long bstride, double *obase, long ostride, int na, const int *act,
                         int n3) {
   if (int e = tp_prepare(h)) return e;
+  const long L = std::max(h->tp_L[0], h->tp_L[1]);
+  if (h->tp_Lcap < L * h->cap) {
+    if (int e = h->tp_Bh.alloc((size_t)L * h->cap * 2)) return e;
+    if (int e = h->tp_Bl.alloc((size_t)L * h->cap * 2)) return e;
+    h->tp_Lcap = L * h->cap;
+  }
+  if (na > h->cap) return fail(PNFAM_GPU_EINVAL, "more active points than the batch capacity");
+  if (h->tp_njobs[stage][ri])
+    hipLaunchKernelGGL(k_tp_bconv, dim3(h->tp_njobs[stage][ri], na), dim3(256), 0, h->stream, terms,
+                       h->d_tp_toff[stage][ri].as<int>(), h->d_tp_jobs[stage][ri].as<int2>(), bbase, bstride, h->nxy,
+                       act, L, h->tp_Bh.as<__bf16>(), h->tp_Bl.as<__bf16>());
   TpMats M{h->tp_rmh.as<__bf16>(), h->tp_rml.as<__bf16>(), h->tp_cmh.as<__bf16>(), h->tp_cml.as<__bf16>(),
            h->tp_mstride};
-  for (int c = 1; c <= TP_MAXCT; ++c) {
-    const int n = h->tp_count[stage][ri][c];
+  TpB BB{h->tp_Bh.as<__bf16>(), h->tp_Bl.as<__bf16>(), L, h->d_tp_toff[stage][ri].as<int>()};
+  // few points: one launch over all tile classes (the widest template), so the
+  // classes' dependency chains overlap instead of running one after another
+  const bool merge = na <= TP_MERGE_N;
+  int cmax = 0, smax = 0, ntot = 0;
+  for (int c = 1; c <= TP_MAXCT; ++c)
+    if (h->tp_count[stage][ri][c]) { cmax = c; smax = std::max(smax, h->tp_strips[stage][ri][c]); ntot += h->tp_count[stage][ri][c]; }
+  for (int c = merge ? cmax : 1; c <= TP_MAXCT && c >= 1; ++c) {
+    const int n = merge ? ntot : h->tp_count[stage][ri][c];
     if (!n) continue;
-    const dim3 grid(n, na, h->tp_strips[stage][ri][c]);
-    const Group *g = groups + h->tp_first[stage][ri][c];
+    const dim3 grid(n, na, merge ? smax : h->tp_strips[stage][ri][c]);
+    const Group *g = groups + (merge ? 0 : h->tp_first[stage][ri][c]);
 #define TB_LAUNCH(NC)                                                                                          \
-  hipLaunchKernelGGL(k_triprod_bf16<NC>, grid, dim3(64), 0, h->stream, g, terms, M, bbase, bstride, obase, ostride, \
-                     act, h->nxy, n3)
+  hipLaunchKernelGGL(k_triprod_bf16<NC>, grid, dim3(64), 0, h->stream, g, terms, M, BB, obase, ostride, act, h->nxy, n3)
     switch (c) {
       case 1: TB_LAUNCH(1); break;
       case 2: TB_LAUNCH(2); break;
@@ -1555,6 +1618,7 @@ int launch_triprod_bf16(pnfam_gpu_ctx *h, int stage, int ri, const Group *groups
       default: TB_LAUNCH(8); break;
     }
 #undef TB_LAUNCH
+    if (merge) break;
   }
   HIP_TRY(hipGetLastError());
   return 0;
@@ -2201,11 +2265,23 @@ int stage_hamiltonian_fac(pnfam_gpu_ctx *h, int na, const int *act, int n3) {
   P.act = act; P.n3 = n3;
   { Prof pr(h, K_FAC_PROJ, na + 2 * n3);
     const Tile2 *tl = h->d_tiles2.as<Tile2>();
+    // few points: one launch per kind over all row-tile classes (widest template)
+    const bool merge = na <= TP_MERGE_N;
+    int mk[2] = {0, 0}, mfirst[2] = {-1, -1}, mcount[2] = {0, 0};
     for (int k = 0; k < 16; ++k) {
-      const int n = h->pj_count[k];
+      if (!h->pj_count[k]) continue;
+      const int kind = k >> 3;
+      mk[kind] = k;
+      if (mfirst[kind] < 0) mfirst[kind] = h->pj_first[k];
+      mcount[kind] += h->pj_count[k];
+    }
+    for (int k = 0; k < 16; ++k) {
+      const int kind = k >> 3;
+      if (merge && k != mk[kind]) continue;
+      const int n = merge ? mcount[kind] : h->pj_count[k];
       if (!n) continue;
       const dim3 grid(n, na);
-      const Tile2 *tk = tl + h->pj_first[k];
+      const Tile2 *tk = tl + (merge ? mfirst[kind] : h->pj_first[k]);
 #define FP_LAUNCH(NRT, KIND) hipLaunchKernelGGL((k_fac_project<NRT, KIND>), grid, dim3(64 * FAC_NW), 0, h->stream, P, tk)
       switch (k) {
         case 1: FP_LAUNCH(1, 0); break;
@@ -2418,6 +2494,8 @@ int pnfam_gpu_set_operator(pnfam_gpu_ctx *h, int32_t beta_minus, const int32_t *
     tp_patch_terms(h, terms);
     if (!tp_terms_ok(terms)) h->tp_bf16 = false;   // A or C not whole blocks: FP64 products only
     int e;
+    h->tp_L[0] = tp_b_layout(h, 0, ri, terms, ri == 0 ? 0 : h->tp_L[0], e);
+    if (e) return e;
     if ((e = upload(h->d_terms_q2s[ri], terms.data(), std::max<size_t>(1, terms.size()) * sizeof(Term), h->stream)) ||
         (e = upload(h->d_groups_q2s[ri], groups.data(), std::max<size_t>(1, groups.size()) * sizeof(Group), h->stream)))
       return e;
@@ -2455,6 +2533,8 @@ int pnfam_gpu_set_operator(pnfam_gpu_ctx *h, int32_t beta_minus, const int32_t *
     tp_patch_terms(h, terms);
     if (!tp_terms_ok(terms)) h->tp_bf16 = false;   // A or C not whole blocks: FP64 products only
     int e;
+    h->tp_L[1] = tp_b_layout(h, 1, ri, terms, ri == 0 ? 0 : h->tp_L[1], e);
+    if (e) return e;
     if ((e = upload(h->d_terms_s2q[ri], terms.data(), std::max<size_t>(1, terms.size()) * sizeof(Term), h->stream)) ||
         (e = upload(h->d_groups_s2q[ri], groups.data(), std::max<size_t>(1, groups.size()) * sizeof(Group), h->stream)))
       return e;
