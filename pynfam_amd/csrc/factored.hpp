@@ -609,30 +609,42 @@ __global__ void k_tp_copy(const double *__restrict__ m, const int4 *__restrict__
   }
 }
 
-// op(B) of every term into the padded [j][k] BF16 hi/lo layout of k_triprod_bf16:
-// one job = up to TP_BJOB elements of one term; blockIdx.y = active point.
-constexpr int TP_BJOB = 2048;
+// op(B) of every distinct B block into the padded [j][k] BF16 hi/lo layout of
+// k_triprod_bf16.  One job = one 16 (j) x 32 (k) tile of one term's region,
+// staged through LDS so that the FP64 reads run along B's storage order
+// (j for a transposed block, k otherwise) and the writes along k.
+constexpr int TP_TJ = 16, TP_TK = 32;
 __global__ void __launch_bounds__(256) k_tp_bconv(const Term *__restrict__ terms, const int *__restrict__ toff,
                                                   const int2 *__restrict__ jobs, const double *__restrict__ bbase,
                                                   long bstride, long nxy, const int *__restrict__ act, long L,
                                                   __bf16 *__restrict__ oh, __bf16 *__restrict__ ol) {
-  const int2 jb = jobs[blockIdx.x];          // (term, first element)
+  __shared__ float tile[TP_TJ][TP_TK + 1];
+  const int2 jb = jobs[blockIdx.x];          // (term, tile)
   const int slot = blockIdx.y;
   const int w = act[slot];
   const Term tm = terms[jb.x];
   const int nda = tm.nda, ndc = tm.ndc;
   const int Pa = (nda + 31) & ~31;
-  const int n = (ndc + 15) / 16 * 16 * Pa;
+  const int ntk = Pa / TP_TK;
+  const int j0 = (jb.y / ntk) * TP_TJ, k0 = (jb.y % ntk) * TP_TK;
   const double *Bb = bbase + (size_t)w * bstride + (size_t)tm.b_slot * nxy + tm.b_off;
-  const size_t o = (size_t)slot * L + toff[jb.x];
-  const int e1 = min(n, jb.y + TP_BJOB);
-  for (int e = jb.y + threadIdx.x; e < e1; e += blockDim.x) {
-    const int j = e / Pa, k = e - (e / Pa) * Pa;
+  for (int idx = threadIdx.x; idx < TP_TJ * TP_TK; idx += blockDim.x) {
+    int jj, kk;
+    if (tm.tb) { jj = idx % TP_TJ; kk = idx / TP_TJ; }
+    else { kk = idx % TP_TK; jj = idx / TP_TK; }
+    const int j = j0 + jj, k = k0 + kk;
     float v = 0.0f;
     if (j < ndc && k < nda) v = (float)(tm.tb ? Bb[j + (size_t)k * tm.ldb] : Bb[k + (size_t)j * tm.ldb]);
+    tile[jj][kk] = v;
+  }
+  __syncthreads();
+  const size_t o = (size_t)slot * L + toff[jb.x];
+  for (int idx = threadIdx.x; idx < TP_TJ * TP_TK; idx += blockDim.x) {
+    const int jj = idx / TP_TK, kk = idx % TP_TK;
     __bf16 h, l;
-    split_bf16(v, h, l);
-    oh[o + e] = h;
-    ol[o + e] = l;
+    split_bf16(tile[jj][kk], h, l);
+    const size_t e = o + (size_t)(j0 + jj) * Pa + k0 + kk;
+    oh[e] = h;
+    ol[e] = l;
   }
 }

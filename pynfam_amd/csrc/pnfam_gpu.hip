@@ -12,6 +12,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
+#include <map>
 #include <chrono>
 #include <cstdlib>
 #include <cmath>
@@ -1537,12 +1539,18 @@ long tp_b_layout(pnfam_gpu_ctx *h, int stage, int ri, const std::vector<Term> &t
   std::vector<int> toff(std::max<size_t>(1, terms.size()), 0);
   std::vector<int2> jobs;
   long off = base;
+  // terms that read the same block of B the same way share one converted copy
+  std::map<std::array<int, 5>, int> seen;
   for (size_t i = 0; i < terms.size(); ++i) {
     const Term &t = terms[i];
-    const long n = (long)((t.ndc + 15) / 16 * 16) * ((t.nda + 31) & ~31);
+    const std::array<int, 5> key{t.b_slot, t.b_off, t.tb, t.nda, t.ndc};
+    const auto it = seen.find(key);
+    if (it != seen.end()) { toff[i] = it->second; continue; }
+    const int nj = (t.ndc + 15) / 16, nk = ((t.nda + 31) & ~31) / TP_TK;
     toff[i] = (int)off;
-    for (long e = 0; e < n; e += TP_BJOB) jobs.push_back(make_int2((int)i, (int)e));
-    off += n;
+    seen[key] = (int)off;
+    for (int q = 0; q < nj * nk; ++q) jobs.push_back(make_int2((int)i, q));
+    off += (long)nj * TP_TJ * nk * TP_TK;
   }
   if (jobs.empty()) jobs.push_back(make_int2(0, 0));
   h->tp_njobs[stage][ri] = terms.empty() ? 0 : (int)jobs.size();
