@@ -498,28 +498,41 @@ struct TpB {
   const int *toff;                       // per term
 };
 
+// One workgroup of TP_NW waves per (group, frequency, 16-row strip): wave j takes
+// the group's terms j, j + TP_NW, ... with its own LDS strip; the waves' partial
+// outputs are summed through LDS at the end.
+constexpr int TP_NW = 4;
+
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <int NCT>
-__global__ void __launch_bounds__(64) k_triprod_bf16(const Group *__restrict__ groups, const Term *__restrict__ terms,
-                                                     TpMats M, TpB BB, double *__restrict__ obase, long ostride,
-                                                     const int *__restrict__ act, long nxy, int n3) {
+__global__ void __launch_bounds__(64 * TP_NW) k_triprod_bf16(const Group *__restrict__ groups,
+                                                             const Term *__restrict__ terms, TpMats M, TpB BB,
+                                                             double *__restrict__ obase, long ostride,
+                                                             const int *__restrict__ act, long nxy, int n3) {
   constexpr int TW = (16 * NCT + 31) / 32 * 32;   // strip width padded to whole K steps
   constexpr int PT = TW + 4;
-  __shared__ __attribute__((aligned(16))) float sT[16 * PT];
+  __shared__ __attribute__((aligned(16))) float smem[TP_NW * 16 * PT];
   const Group g = groups[blockIdx.x];
   const int strip = blockIdx.z;
   const int w = act[blockIdx.y];
   const bool three = (int)blockIdx.y < n3;
   const int i0 = strip * 16;
-  if (i0 >= g.nrow) return;                       // uniform per wave
-  const int lane = threadIdx.x, fi = lane & 15, fg = lane >> 4;
+  if (i0 >= g.nrow) return;                       // uniform per workgroup
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int fi = lane & 15, fg = lane >> 4;
+  float *sT = smem + wave * 16 * PT;
   for (int e = lane; e < 16 * PT; e += 64) sT[e] = 0.0f;   // padding columns stay zero
-  double *O = obase + (size_t)w * ostride + (size_t)g.out_slot * nxy + g.out_off;
   const v4f z4 = {0.f, 0.f, 0.f, 0.f};
   v4f acc[NCT];
 #pragma unroll
   for (int c = 0; c < NCT; ++c) acc[c] = z4;
-  __syncthreads();
-  for (int t = 0; t < g.count; ++t) {
+  wave_sync();
+  for (int t = wave; t < g.count; t += TP_NW) {
     const Term tm = terms[g.first + t];
     const int nda = tm.nda, ndc = tm.ndc;
     const int Pa = (nda + 31) & ~31, Pc = (ndc + 31) & ~31;
@@ -550,12 +563,12 @@ __global__ void __launch_bounds__(64) k_triprod_bf16(const Group *__restrict__ g
         T[c] = mfma_split(ah, al, bh, bl, T[c], three);
       }
     }
-    // park s T: D[4 fg + v][fi] of column tile c
+    // park s T in this wave's strip: D[4 fg + v][fi] of column tile c
 #pragma unroll
     for (int c = 0; c < NCT; ++c)
 #pragma unroll
       for (int v = 0; v < 4; ++v) sT[(4 * fg + v) * PT + c * 16 + fi] = sgn * T[c][v];
-    __syncthreads();
+    wave_sync();
     // phase 2: acc += T op(C)
     for (int l0 = 0; l0 < ndc; l0 += 32) {
       const float4 t0 = *(const float4 *)&sT[fi * PT + l0 + 8 * fg];
@@ -577,8 +590,28 @@ __global__ void __launch_bounds__(64) k_triprod_bf16(const Group *__restrict__ g
         acc[c] = mfma_split(th, tl, ch, cl, acc[c], three);
       }
     }
-    __syncthreads();
+    wave_sync();   // the strip is rewritten by the next term
   }
+  // sum the waves: waves 1.. park their tiles, wave 0 adds and writes
+  __syncthreads();
+  if (wave > 0) {
+    float *red = smem + (size_t)(wave - 1) * NCT * 4 * 64;
+#pragma unroll
+    for (int c = 0; c < NCT; ++c)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) red[(c * 4 + v) * 64 + lane] = acc[c][v];
+  }
+  __syncthreads();
+  if (wave != 0) return;
+#pragma unroll
+  for (int j = 0; j < TP_NW - 1; ++j) {
+    const float *red = smem + (size_t)j * NCT * 4 * 64;
+#pragma unroll
+    for (int c = 0; c < NCT; ++c)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) acc[c][v] += red[(c * 4 + v) * 64 + lane];
+  }
+  double *O = obase + (size_t)w * ostride + (size_t)g.out_slot * nxy + g.out_off;
 #pragma unroll
   for (int c = 0; c < NCT; ++c) {
     const int j = c * 16 + fi;

@@ -33,7 +33,58 @@ constexpr int GLDP = 40;          // LDS row pitch in bf16 (80 bytes)
 __constant__ int8_t kProjZ[18][2] = {{0, 0}, {0, 1}, {1, 0}, {0, 2}, {2, 0}, {0, 3}, {3, 0}, {0, 4}, {4, 0},
                                      {1, 1}, {1, 2}, {2, 1}, {1, 3}, {3, 1}, {2, 2}, {2, 3}, {3, 2}, {3, 3}};
 constexpr int NPROJ_C = 18;
-constexpr int NDENS_C = 16;   // (x, y) in {0..3}^2, c = 4 x + y
+
+// Density tables: per spin class the 28 densities need only nine linear
+// combinations of the products psi_x(r,a) psi_y(r,b) (x, y = psi, d_r, d_phi,
+// d_z psi; the products of pnfam_hamiltonian_blas.f90:255-581), because the spin
+// signs are constant over a class.  kDensW[cls][c][4 x + y]: classes 0 (up,up)
+// and 3 (down,down) use the same-spin combinations P, classes 1 (up,down) and 2
+// (down,up) the opposite-spin combinations Q with e = +1 / -1 baked in:
+//   P: 00, 02+20, 11+22+33, 01-10, 03-30, 03+30, 13+31, 23-32, 33
+//   Q: 00, 02+20, e(02-20)+01+10, 11+22+33, 01-10, 03-30, 2*11+e(12-21),
+//      2e*22+21-12, 31+13+e(32-23)
+constexpr int NDENS_C = 9;
+__constant__ int8_t kDensW[4][NDENS_C][16] = {
+    // class 0 (up, up): P
+    {{1, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0},
+     {0, 0, 1, 0, 0, 0, 0, 0, 1, 0, 0, 0, 0, 0, 0, 0},
+     {0, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1},
+     {0, 1, 0, 0, -1, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0},
+     {0, 0, 0, 1, 0, 0, 0, 0, 0, 0, 0, 0, -1, 0, 0, 0},
+     {0, 0, 0, 1, 0, 0, 0, 0, 0, 0, 0, 0, 1, 0, 0, 0},
+     {0, 0, 0, 0, 0, 0, 0, 1, 0, 0, 0, 0, 0, 1, 0, 0},
+     {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1, 0, 0, -1, 0},
+     {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1}},
+    // class 1 (up, down): Q with e = +1
+    {{1, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0},
+     {0, 0, 1, 0, 0, 0, 0, 0, 1, 0, 0, 0, 0, 0, 0, 0},
+     {0, 1, 1, 0, 1, 0, 0, 0, -1, 0, 0, 0, 0, 0, 0, 0},
+     {0, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1},
+     {0, 1, 0, 0, -1, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0},
+     {0, 0, 0, 1, 0, 0, 0, 0, 0, 0, 0, 0, -1, 0, 0, 0},
+     {0, 0, 0, 0, 0, 2, 1, 0, 0, -1, 0, 0, 0, 0, 0, 0},
+     {0, 0, 0, 0, 0, 0, -1, 0, 0, 1, 2, 0, 0, 0, 0, 0},
+     {0, 0, 0, 0, 0, 0, 0, 1, 0, 0, 0, -1, 0, 1, 1, 0}},
+    // class 2 (down, up): Q with e = -1
+    {{1, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0},
+     {0, 0, 1, 0, 0, 0, 0, 0, 1, 0, 0, 0, 0, 0, 0, 0},
+     {0, 1, -1, 0, 1, 0, 0, 0, 1, 0, 0, 0, 0, 0, 0, 0},
+     {0, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1},
+     {0, 1, 0, 0, -1, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0},
+     {0, 0, 0, 1, 0, 0, 0, 0, 0, 0, 0, 0, -1, 0, 0, 0},
+     {0, 0, 0, 0, 0, 2, -1, 0, 0, 1, 0, 0, 0, 0, 0, 0},
+     {0, 0, 0, 0, 0, 0, -1, 0, 0, 1, -2, 0, 0, 0, 0, 0},
+     {0, 0, 0, 0, 0, 0, 0, 1, 0, 0, 0, 1, 0, 1, -1, 0}},
+    // class 3 (down, down): P
+    {{1, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0},
+     {0, 0, 1, 0, 0, 0, 0, 0, 1, 0, 0, 0, 0, 0, 0, 0},
+     {0, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1},
+     {0, 1, 0, 0, -1, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0},
+     {0, 0, 0, 1, 0, 0, 0, 0, 0, 0, 0, 0, -1, 0, 0, 0},
+     {0, 0, 0, 1, 0, 0, 0, 0, 0, 0, 0, 0, 1, 0, 0, 0},
+     {0, 0, 0, 0, 0, 0, 0, 1, 0, 0, 0, 0, 0, 1, 0, 0},
+     {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1, 0, 0, -1, 0},
+     {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1}}};
 
 __device__ inline void split_bf16(float v, __bf16 &hi, __bf16 &lo) {
   hi = (__bf16)v;
@@ -44,6 +95,7 @@ __device__ inline void split_bf16(float v, __bf16 &hi, __bf16 &lo) {
 struct TabArgs {
   const double *psi[5];       // [state][r]
   const int *sa, *sb;         // [Kp] states of the element, -1 for padding
+  int kbeg[5];                // class ranges of the element order
   int Kp, nghl, RP;
   int nc;                     // combos
   int dens;                   // 1: density layout [c][r][k], 0: projection layout [c][k][r]
@@ -57,7 +109,7 @@ __global__ void k_mix_tables(TabArgs T) {
   const int c = blockIdx.y;
   int x, y;
   if (T.single) { x = 0; y = 0; }
-  else if (T.dens) { x = c >> 2; y = c & 3; }
+  else if (T.dens) { x = -1; y = -1; }       // combinations kDensW
   else { x = kProjZ[c][0]; y = kProjZ[c][1]; }
   const size_t n = (size_t)T.Kp * T.RP;
   for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
@@ -66,8 +118,21 @@ __global__ void k_mix_tables(TabArgs T) {
     else { r = (int)(e % T.RP); k = (int)(e / T.RP); }
     const int a = T.sa[k], b = T.sb[k];
     float v = 0.0f;
-    if (a >= 0 && r < T.nghl)
-      v = (float)(T.psi[x][(size_t)a * T.nghl + r] * T.psi[y][(size_t)b * T.nghl + r]);
+    if (a >= 0 && r < T.nghl) {
+      if (x >= 0) {
+        v = (float)(T.psi[x][(size_t)a * T.nghl + r] * T.psi[y][(size_t)b * T.nghl + r]);
+      } else {
+        const int cls = k < T.kbeg[1] ? 0 : k < T.kbeg[2] ? 1 : k < T.kbeg[3] ? 2 : 3;
+        double pa[4], pb[4];
+        for (int q = 0; q < 4; ++q) {
+          pa[q] = T.psi[q][(size_t)a * T.nghl + r];
+          pb[q] = T.psi[q][(size_t)b * T.nghl + r];
+        }
+        double acc = 0.0;
+        for (int xy = 0; xy < 16; ++xy) acc += (double)kDensW[cls][c][xy] * pa[xy >> 2] * pb[xy & 3];
+        v = (float)acc;
+      }
+    }
     __bf16 h, l;
     split_bf16(v, h, l);
     // density layout [c][r][k]; projection layout [k][c][r] (K = (c, r) contiguous per row k)
@@ -85,19 +150,31 @@ __global__ void k_fill_f32(float *__restrict__ p, size_t n) {
 
 // Projection results O[k][n] (FP32, n fastest) into the FP64 block matrices:
 // column n = (slot * nsec + sec) * 2 + ri, element elem[k] of quadrant q(sec).
-__global__ void k_mix_scatter(const float *__restrict__ O, long ldO, const int *__restrict__ elem, int Kp, int nS,
-                              int nsec, int q0, int q1, const int *__restrict__ act, double *__restrict__ hsp,
-                              long nxy) {
-  const int s = blockIdx.y * 64 + (threadIdx.x & 63);
-  if (s >= nS) return;
-  const int ri = s & 1;
-  const int sec = nsec == 2 ? (s >> 1) & 1 : 0;
-  const int slot = s / (2 * nsec);
-  const int q = sec ? q1 : q0;
-  double *out = hsp + ((size_t)act[slot] * 8 + 2 * q + ri) * nxy;
-  for (int k = blockIdx.x * 4 + (threadIdx.x >> 6); k < Kp; k += gridDim.x * 4) {
+// A 64 (k) x 64 (n) tile goes through LDS: the reads run along n, the writes
+// along k (consecutive elements of a block column).
+__global__ void __launch_bounds__(256) k_mix_scatter(const float *__restrict__ O, long ldO,
+                                                     const int *__restrict__ elem, int Kp, int nS, int nsec, int q0,
+                                                     int q1, const int *__restrict__ act, double *__restrict__ hsp,
+                                                     long nxy) {
+  __shared__ float tile[64][65];
+  const int k0 = blockIdx.x * 64, s0 = blockIdx.y * 64;
+  for (int idx = threadIdx.x; idx < 64 * 64; idx += 256) {
+    const int kk = idx >> 6, ss = idx & 63;
+    const int k = k0 + kk, s = s0 + ss;
+    tile[kk][ss] = (k < Kp && s < nS) ? O[(size_t)k * ldO + s] : 0.0f;
+  }
+  __syncthreads();
+  for (int idx = threadIdx.x; idx < 64 * 64; idx += 256) {
+    const int ss = idx >> 6, kk = idx & 63;
+    const int k = k0 + kk, s = s0 + ss;
+    if (k >= Kp || s >= nS) continue;
     const int e = elem[k];
-    if (e >= 0) out[e] = (double)O[(size_t)k * ldO + s];
+    if (e < 0) continue;
+    const int ri = s & 1;
+    const int sec = nsec == 2 ? (s >> 1) & 1 : 0;
+    const int slot = s / (2 * nsec);
+    const int q = sec ? q1 : q0;
+    hsp[((size_t)act[slot] * 8 + 2 * q + ri) * nxy + e] = (double)tile[kk][ss];
   }
 }
 
@@ -380,20 +457,48 @@ __global__ void __launch_bounds__(128) k_mix_dens(CombArgs A) {
   const size_t plane_k = (size_t)A.RP * A.ldk;
   const float *Ok = sec ? A.Ok1 : A.Ok0;
   for (int sb = 0; sb < 2; ++sb) {
-    const int s = sb == 0 ? 1 : -1;
+    const double s = sb == 0 ? 1.0 : -1.0, e = -s;
     const int cd = 2 * sb + sb, co = 2 * (1 - sb) + sb;   // (s_a = s_b, s_b), (s_a != s_b, s_b)
-    double Pr[4][4], Pi[4][4], Qr[4][4], Qi[4][4];
+    double Pr9[NDENS_C], Pi9[NDENS_C], Qr9[NDENS_C], Qi9[NDENS_C];
 #pragma unroll
-    for (int x = 0; x < 4; ++x)
-#pragma unroll
-      for (int y = 0; y < 4; ++y) {
-        const int c = 4 * x + y;
-        const float *pd = A.Of + (size_t)(cd * NDENS_C + c) * plane_f + (size_t)r * A.ldf + 2 * j;
-        const float *po = A.Of + (size_t)(co * NDENS_C + c) * plane_f + (size_t)r * A.ldf + 2 * j;
-        Pr[x][y] = pd[0]; Pi[x][y] = pd[1];
-        Qr[x][y] = po[0]; Qi[x][y] = po[1];
-      }
-    density_accumulate(acc, Pr, Pi, Qr, Qi, s);
+    for (int c = 0; c < NDENS_C; ++c) {
+      const float *pd = A.Of + (size_t)(cd * NDENS_C + c) * plane_f + (size_t)r * A.ldf + 2 * j;
+      const float *po = A.Of + (size_t)(co * NDENS_C + c) * plane_f + (size_t)r * A.ldf + 2 * j;
+      Pr9[c] = pd[0]; Pi9[c] = pd[1];
+      Qr9[c] = po[0]; Qi9[c] = po[1];
+    }
+#define ACC(d, ri) acc[2 * (d) + (ri)]
+    // same spin (P combinations); i*z adds (-Im z, Re z)
+    ACC(D_RHO, 0) += Pr9[0]; ACC(D_RHO, 1) += Pi9[0];
+    ACC(D_SZ, 0) += s * Pr9[0]; ACC(D_SZ, 1) += s * Pi9[0];
+    ACC(D_JP, 0) += 0.5 * Pr9[1]; ACC(D_JP, 1) += 0.5 * Pi9[1];
+    ACC(D_TJPZ, 0) += 0.5 * s * Pr9[1]; ACC(D_TJPZ, 1) += 0.5 * s * Pi9[1];
+    ACC(D_TAU, 0) += Pr9[2]; ACC(D_TAU, 1) += Pi9[2];
+    ACC(D_TZ, 0) += s * Pr9[2]; ACC(D_TZ, 1) += s * Pi9[2];
+    ACC(D_JR, 0) -= 0.5 * Pi9[3]; ACC(D_JR, 1) += 0.5 * Pr9[3];
+    ACC(D_TJRZ, 0) -= 0.5 * s * Pi9[3]; ACC(D_TJRZ, 1) += 0.5 * s * Pr9[3];
+    ACC(D_JZ, 0) -= 0.5 * Pi9[4]; ACC(D_JZ, 1) += 0.5 * Pr9[4];
+    ACC(D_TJZZ, 0) -= 0.5 * s * Pi9[4]; ACC(D_TJZZ, 1) += 0.5 * s * Pr9[4];
+    ACC(D_GS, 0) += s * Pr9[5]; ACC(D_GS, 1) += s * Pi9[5];
+    ACC(D_FR, 0) += 0.5 * s * Pr9[6]; ACC(D_FR, 1) += 0.5 * s * Pi9[6];
+    ACC(D_FP, 0) -= 0.5 * s * Pi9[7]; ACC(D_FP, 1) += 0.5 * s * Pr9[7];
+    ACC(D_FZ, 0) += s * Pr9[8]; ACC(D_FZ, 1) += s * Pi9[8];
+    // opposite spin (Q combinations, e = -s)
+    ACC(D_SR, 0) += Qr9[0]; ACC(D_SR, 1) += Qi9[0];
+    ACC(D_SP, 0) -= e * Qi9[0]; ACC(D_SP, 1) += e * Qr9[0];
+    ACC(D_TJPR, 0) += 0.5 * Qr9[1]; ACC(D_TJPR, 1) += 0.5 * Qi9[1];
+    ACC(D_TJPP, 0) -= 0.5 * e * Qi9[1]; ACC(D_TJPP, 1) += 0.5 * e * Qr9[1];
+    ACC(D_GS, 0) += Qr9[2]; ACC(D_GS, 1) += Qi9[2];
+    ACC(D_TR, 0) += Qr9[3]; ACC(D_TR, 1) += Qi9[3];
+    ACC(D_TP, 0) -= e * Qi9[3]; ACC(D_TP, 1) += e * Qr9[3];
+    ACC(D_TJRR, 0) -= 0.5 * Qi9[4]; ACC(D_TJRR, 1) += 0.5 * Qr9[4];
+    ACC(D_TJRP, 0) -= 0.5 * e * Qr9[4]; ACC(D_TJRP, 1) -= 0.5 * e * Qi9[4];
+    ACC(D_TJZR, 0) -= 0.5 * Qi9[5]; ACC(D_TJZR, 1) += 0.5 * Qr9[5];
+    ACC(D_TJZP, 0) -= 0.5 * e * Qr9[5]; ACC(D_TJZP, 1) -= 0.5 * e * Qi9[5];
+    ACC(D_FR, 0) += 0.5 * Qr9[6]; ACC(D_FR, 1) += 0.5 * Qi9[6];
+    ACC(D_FP, 0) -= 0.5 * Qi9[7]; ACC(D_FP, 1) += 0.5 * Qr9[7];
+    ACC(D_FZ, 0) += 0.5 * Qr9[8]; ACC(D_FZ, 1) += 0.5 * Qi9[8];
+#undef ACC
     // pairing densities (pnfam_hamiltonian_blas.f90:584-640)
     const float *kd = Ok + (size_t)cd * plane_k + (size_t)r * A.ldk + 2 * slot;
     const float *ko = Ok + (size_t)co * plane_k + (size_t)r * A.ldk + 2 * slot;

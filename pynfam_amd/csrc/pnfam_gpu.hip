@@ -1007,20 +1007,8 @@ __global__ void k_green(GreenArgs A) {
 }
 
 // ---- mixed path: the hamiltonian is linear in vin, so H(vin_k) is carried as
-// an FP64 sum of H applied to the increments vin_k - vin_{k-1} --------------
-// dvin = vin - vprev, vprev = vin (per active frequency, n doubles each)
-__global__ void k_delta(const double *__restrict__ vin, double *__restrict__ vprev, double *__restrict__ dvin,
-                        long n, const int *__restrict__ act) {
-  const size_t w = act[blockIdx.y];
-  const double *a = vin + w * n;
-  double *p = vprev + w * n, *d = dvin + w * n;
-  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
-    const double v = a[e];
-    d[e] = v - p[e];
-    p[e] = v;
-  }
-}
-
+// an FP64 sum of H applied to the increments vin_k - vin_{k-1}; the increment
+// is the step the Broyden update adds (k_bro_update writes it) --------------
 // hacc = hqp (set != 0, a full evaluation) or hacc += hqp (an increment)
 __global__ void k_hacc(double *__restrict__ hacc, const double *__restrict__ hqp, long n, int set,
                        const int *__restrict__ act) {
@@ -1029,14 +1017,6 @@ __global__ void k_hacc(double *__restrict__ hacc, const double *__restrict__ hqp
   const double *b = hqp + w * n;
   for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x)
     a[e] = set ? b[e] : a[e] + b[e];
-}
-
-// vprev = vin (the full evaluation just used vin)
-__global__ void k_copy_act(const double *__restrict__ src, double *__restrict__ dst, long n,
-                           const int *__restrict__ act) {
-  const size_t w = act[blockIdx.y];
-  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x)
-    dst[w * n + e] = src[w * n + e];
 }
 
 // ---- Broyden (pnfam_broyden.f90:117-216), per frequency, FP64 ---------------
@@ -1052,6 +1032,7 @@ struct BroArgs {
   double *si;             // [w]
   double *vmax;           // [w] max |vin|
   double *scal;           // [w][4]: normi, ...
+  double *dvin;           // mixed path: [w][N] the step added to vin (the next increment); null for FP64
   long N;
   int M, nwg, iter, ipos, inext, iter_used;
   double alpha;
@@ -1261,19 +1242,26 @@ __global__ void __launch_bounds__(BRO_WG) k_bro_update(BroArgs B, int mode) {
   double *vi = B.vin + (size_t)w * N;
   const double *vo = B.vout + (size_t)w * N;
   const double *gm = B.coef + (size_t)w * (B.M > 0 ? B.M : 1);
+  double *dv_out = B.dvin ? B.dvin + (size_t)w * N : nullptr;
   for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < N; e += (long)gridDim.x * blockDim.x) {
     const double r = vo[e];
-    if (mode == 2) { vi[e] += r; continue; }
-    if (mode == 1) { vi[e] += B.alpha * r; continue; }
-    double c = B.alpha * r;
-    for (int i = 0; i < B.iter_used; ++i) {
-      const size_t o = ((size_t)w * B.M + i) * N + e;
-      c -= gm[i] * (B.dv[o] + B.alpha * B.df[o]);
+    double c;
+    if (mode == 2) {
+      c = r;
+    } else if (mode == 1) {
+      c = B.alpha * r;
+    } else {
+      c = B.alpha * r;
+      for (int i = 0; i < B.iter_used; ++i) {
+        const size_t o = ((size_t)w * B.M + i) * N + e;
+        c -= gm[i] * (B.dv[o] + B.alpha * B.df[o]);
+      }
+      const size_t on = ((size_t)w * B.M + B.inext) * N + e;
+      B.df[on] = r;
+      B.dv[on] = vi[e];
     }
-    const size_t on = ((size_t)w * B.M + B.inext) * N + e;
-    B.df[on] = r;
-    B.dv[on] = vi[e];
     vi[e] += c;
+    if (dv_out) dv_out[e] = c;
   }
 }
 
@@ -1342,8 +1330,8 @@ struct pnfam_gpu_ctx {
   int hist_cap = -1;
   DevPtr d_vin, d_vout, d_rsp, d_hsp, d_hqp, d_dens, d_flds, d_part, d_cpart, d_S, d_si, d_vmax, d_scal;
   DevPtr d_df, d_dv, d_gram, d_coef, d_wre, d_wim, d_act;
-  // mixed path: last vin fed to the hamiltonian, its increment, accumulated FP64 H20/H02
-  DevPtr d_act64, d_actlo, d_vprev, d_dvin, d_hacc;
+  // mixed path: the next increment of vin, accumulated FP64 H20/H02
+  DevPtr d_act64, d_actlo, d_dvin, d_hacc;
   // split-BF16 grid stage: one element layout per block structure (the mean-field
   // set in map f, one pairing set per sector), pair tables and operands
   struct MixSet {
@@ -1614,7 +1602,8 @@ int launch_triprod_bf16(pnfam_gpu_ctx *h, int stage, int ri, const Group *groups
     const dim3 grid(n, na, merge ? smax : h->tp_strips[stage][ri][c]);
     const Group *g = groups + (merge ? 0 : h->tp_first[stage][ri][c]);
 #define TB_LAUNCH(NC)                                                                                          \
-  hipLaunchKernelGGL(k_triprod_bf16<NC>, grid, dim3(64), 0, h->stream, g, terms, M, BB, obase, ostride, act, h->nxy, n3)
+  hipLaunchKernelGGL(k_triprod_bf16<NC>, grid, dim3(64 * TP_NW), 0, h->stream, g, terms, M, BB, obase, ostride, act, \
+                     h->nxy, n3)
     switch (c) {
       case 1: TB_LAUNCH(1); break;
       case 2: TB_LAUNCH(2); break;
@@ -1661,7 +1650,6 @@ int ensure_workspace(pnfam_gpu_ctx *h, int npts, int M) {
   if ((e = h->d_act.alloc(n * 4))) return e;
   if ((e = h->d_act64.alloc(n * 4))) return e;
   if ((e = h->d_actlo.alloc(n * 4))) return e;
-  if ((e = h->d_vprev.alloc(n * 4 * nxy * 8))) return e;
   if ((e = h->d_dvin.alloc(n * 4 * nxy * 8))) return e;
   if ((e = h->d_hacc.alloc(n * 4 * nxy * 8))) return e;
   h->cap = (int)n;
@@ -1824,6 +1812,7 @@ int mix_build_set(pnfam_gpu_ctx *h, pnfam_gpu_ctx::MixSet &S, const BlockMap &m,
   T.psi[0] = h->d_wf.as<double>(); T.psi[1] = h->d_wfdr.as<double>(); T.psi[2] = h->d_wfdp.as<double>();
   T.psi[3] = h->d_wfdz.as<double>(); T.psi[4] = h->d_wfd2.as<double>();
   T.sa = S.d_sa.as<int>(); T.sb = S.d_sb.as<int>();
+  for (int c = 0; c < 5; ++c) T.kbeg[c] = S.kbeg[c];
   T.Kp = S.Kp; T.nghl = h->nghl; T.RP = h->RP; T.single = ntp == 1;
   T.nc = ntp; T.dens = 0; T.hi = S.TPh.as<__bf16>(); T.lo = S.TPl.as<__bf16>();
   hipLaunchKernelGGL(k_mix_tables, dim3(2048, ntp), dim3(256), 0, h->stream, T);
@@ -2097,7 +2086,7 @@ int stage_hamiltonian_mixed(pnfam_gpu_ctx *h, int na, const int *act, int n3) {
                                O + (size_t)k0 * ncol, ncol, (si == 0 ? 4 : 2) * (int64_t)n3))
           return e;
       }
-      hipLaunchKernelGGL(k_mix_scatter, dim3((unsigned)std::min(2048, (S.Kp + 3) / 4), (unsigned)((nS + 63) / 64)),
+      hipLaunchKernelGGL(k_mix_scatter, dim3((unsigned)((S.Kp + 63) / 64), (unsigned)((nS + 63) / 64)),
                          dim3(256), 0, h->stream, O, ncol, S.d_elem.as<int>(), S.Kp, (int)nS, si == 0 ? 2 : 1,
                          si == 0 ? 0 : (si == 1 ? 1 : 2), 3, act, h->d_hsp.as<double>(), h->nxy);
     }
@@ -2850,9 +2839,6 @@ int pnfam_gpu_solve_batch(pnfam_gpu_ctx *h, int32_t npts, const double *re_w, co
   double tau = 1e-3;   // one-product threshold of the mixed path
   if (const char *ev = std::getenv("PNFAM_GPU_MIX_TAU")) tau = std::atof(ev);
   for (int w = 0; w < npts; ++w) { iters[w] = -1; re_s[w] = 0.0; im_s[w] = 0.0; }
-  if (mixed && !skip_res)
-    hipLaunchKernelGGL(k_fill, grid1((size_t)npts * N, 256), dim3(256), 0, st, h->d_vprev.as<double>(),
-                       (size_t)npts * N, 0.0);
   const int nwg = h->bro_nwg;
   const long nq = 4 * nxy;   // doubles of H20/H02 (re, im) per frequency
   // PNFAM_GPU_TRACE=1: one line per iteration on stderr (active points, BF16
@@ -2895,8 +2881,6 @@ int pnfam_gpu_solve_batch(pnfam_gpu_ctx *h, int32_t npts, const double *re_w, co
         Prof pr(h, K_MIXACC, n64);
         hipLaunchKernelGGL(k_hacc, dim3(256, n64), dim3(256), 0, st, h->d_hacc.as<double>(), h->d_hqp.as<double>(),
                            nq, 1, d64);
-        hipLaunchKernelGGL(k_copy_act, dim3(256, n64), dim3(256), 0, st, h->d_vin.as<double>(),
-                           h->d_vprev.as<double>(), N, d64);
       }
     }
     if (use_h && !actlo.empty()) {
@@ -2912,9 +2896,6 @@ int pnfam_gpu_solve_batch(pnfam_gpu_ctx *h, int32_t npts, const double *re_w, co
       const int nlo = (int)actlo.size();
       int *dlo = h->d_actlo.as<int>();
       HIP_TRY(hipMemcpyAsync(dlo, actlo.data(), nlo * 4, hipMemcpyHostToDevice, st));
-      { Prof pr(h, K_MIXACC, nlo);
-        hipLaunchKernelGGL(k_delta, dim3(256, nlo), dim3(256), 0, st, h->d_vin.as<double>(), h->d_vprev.as<double>(),
-                           h->d_dvin.as<double>(), N, dlo); }
       const bool tpl = h->tp_bf16 && !h->basic_kernels;
       if (int e = stage_qp2sp(h, h->d_dvin.as<double>(), nlo, dlo, tpl, n3)) return e;
       if (int e = (h->basic_kernels ? stage_hamiltonian(h, nlo, dlo) : stage_hamiltonian_lowp(h, nlo, dlo, n3)))
@@ -2939,6 +2920,7 @@ int pnfam_gpu_solve_batch(pnfam_gpu_ctx *h, int32_t npts, const double *re_w, co
     B.dv = h->d_dv.as<double>(); B.part = h->d_part.as<double>(); B.gram = h->d_gram.as<double>();
     B.coef = h->d_coef.as<double>(); B.si = h->d_si.as<double>(); B.scal = h->d_scal.as<double>();
     B.vmax = h->d_vmax.as<double>();
+    B.dvin = mixed ? h->d_dvin.as<double>() : nullptr;
     B.N = N; B.M = std::max(M, 0); B.nwg = nwg; B.iter = iter; B.alpha = o->alpha_mix; B.act = dact;
     int mode = 0;
     if (M < 0) mode = 2;
