@@ -509,18 +509,25 @@ __device__ inline void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// The grid is one-dimensional: workgroup L runs on XCD L % 8, and the strips of
+// one (group, point) unit get consecutive ids on one XCD, so they read the
+// unit's B operand through the same L2.
 template <int NCT>
-__global__ void __launch_bounds__(64 * TP_NW) k_triprod_bf16(const Group *__restrict__ groups,
-                                                             const Term *__restrict__ terms, TpMats M, TpB BB,
-                                                             double *__restrict__ obase, long ostride,
-                                                             const int *__restrict__ act, long nxy, int n3) {
+__global__ void __launch_bounds__(64 * TP_NW) k_triprod_bf16(const Group *__restrict__ groups, int ngroups,
+                                                             int nstrip, const Term *__restrict__ terms, TpMats M,
+                                                             TpB BB, double *__restrict__ obase, long ostride,
+                                                             const int *__restrict__ act, int na, long nxy, int n3) {
   constexpr int TW = (16 * NCT + 31) / 32 * 32;   // strip width padded to whole K steps
   constexpr int PT = TW + 4;
   __shared__ __attribute__((aligned(16))) float smem[TP_NW * 16 * PT];
-  const Group g = groups[blockIdx.x];
-  const int strip = blockIdx.z;
-  const int w = act[blockIdx.y];
-  const bool three = (int)blockIdx.y < n3;
+  const int xcd = blockIdx.x & 7, loc = blockIdx.x >> 3;
+  const int strip = loc % nstrip;
+  const int unit = (loc / nstrip) * 8 + xcd;
+  if (unit >= ngroups * na) return;                // uniform per workgroup
+  const int gi = unit % ngroups, slot = unit / ngroups;
+  const Group g = groups[gi];
+  const int w = act[slot];
+  const bool three = slot < n3;
   const int i0 = strip * 16;
   if (i0 >= g.nrow) return;                       // uniform per workgroup
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -543,7 +550,7 @@ __global__ void __launch_bounds__(64 * TP_NW) k_triprod_bf16(const Group *__rest
     // op(C) columns contiguous in l: RM for a transposed C, CM otherwise
     const __bf16 *Ch = (tm.tc ? M.rmh : M.cmh) + mc + 8 * fg;
     const __bf16 *Cl = (tm.tc ? M.rml : M.cml) + mc + 8 * fg;
-    const size_t bo = (size_t)blockIdx.y * BB.L + BB.toff[g.first + t] + (size_t)fi * Pa + 8 * fg;
+    const size_t bo = (size_t)slot * BB.L + BB.toff[g.first + t] + (size_t)fi * Pa + 8 * fg;
     const __bf16 *Bh = BB.h + bo, *Bl = BB.l + bo;
     const float sgn = (float)tm.s;
     // column tiles of this term (a launch over several tile classes uses the

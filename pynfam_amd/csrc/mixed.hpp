@@ -28,11 +28,68 @@ constexpr int GS_ROWS = 256;      // small-operand rows (columns n) per workgrou
 constexpr int GKS = 32;           // K per LDS stage (one MFMA K step)
 constexpr int GLDP = 40;          // LDS row pitch in bf16 (80 bytes)
 
-// The 18 table pairs (z, z') with non-zero mean fields (the assignments of
-// pnfam_hamiltonian_blas.f90:774-1093 never fill z or z' = 4 together with 1..4).
-__constant__ int8_t kProjZ[18][2] = {{0, 0}, {0, 1}, {1, 0}, {0, 2}, {2, 0}, {0, 3}, {3, 0}, {0, 4}, {4, 0},
-                                     {1, 1}, {1, 2}, {2, 1}, {1, 3}, {3, 1}, {2, 2}, {2, 3}, {3, 2}, {3, 3}};
-constexpr int NPROJ_C = 18;
+// Projection tables: the mean fields mf(z, z', s_a, s_b) of
+// pnfam_hamiltonian_blas.f90:774-1093 are non-zero for 18 pairs (z, z') (z or
+// z' = 4 only with 0), and within a spin class several are equal or differ by
+// a sign, so sum_{z,z'} psi_z(a) psi_z'(b) mf needs 15 (same-spin classes) or
+// 16 (opposite-spin classes) table combinations, each with a combined field:
+//   c 0..6: (0,0) (0,1) (1,0) (0,2) (2,0) (0,3) (3,0)      fields mf of the pair
+//   c 7:    (0,4) + (4,0)                                  mf(0,4)
+//   c 8:    (1,1) + (2,2) + (3,3)                          same spin mf(1,1), else mf(3,3)
+//   c 9:    same spin (3,3), else (1,1)                    mf(3,3) - mf(1,1), mf(1,1) - mf(3,3)
+//   c 10:   (1,2) + (2,1), else (1,2) - (2,1)              mf(1,2)
+//   c 11..14: (1,3) (3,1) (2,3) (3,2)                      mf of the pair
+//   c 15:   none, else (2,2)                               -, mf(2,2) - mf(3,3)
+constexpr int NPROJ_C = 16;
+__device__ inline bool spin_flip_class(int cls) { return cls == 1 || cls == 2; }
+__device__ inline double proj_table(int cls, int c, const double (&pa)[5], const double (&pb)[5]) {
+  const bool flip = spin_flip_class(cls);
+  switch (c) {
+    case 0: return pa[0] * pb[0];
+    case 1: return pa[0] * pb[1];
+    case 2: return pa[1] * pb[0];
+    case 3: return pa[0] * pb[2];
+    case 4: return pa[2] * pb[0];
+    case 5: return pa[0] * pb[3];
+    case 6: return pa[3] * pb[0];
+    case 7: return pa[0] * pb[4] + pa[4] * pb[0];
+    case 8: return pa[1] * pb[1] + pa[2] * pb[2] + pa[3] * pb[3];
+    case 9: return flip ? pa[1] * pb[1] : pa[3] * pb[3];
+    case 10: return flip ? pa[1] * pb[2] - pa[2] * pb[1] : pa[1] * pb[2] + pa[2] * pb[1];
+    case 11: return pa[1] * pb[3];
+    case 12: return pa[3] * pb[1];
+    case 13: return pa[2] * pb[3];
+    case 14: return pa[3] * pb[2];
+    default: return flip ? pa[2] * pb[2] : 0.0;
+  }
+}
+// Field of combination c for class cls (component ri), from the fields
+// [208][nghl] of one sector at grid point r.
+__device__ inline double proj_field(int cls, int c, const double *f, int G, int r, int ri) {
+  const int sa = cls >> 1, sb = cls & 1;
+  const bool flip = spin_flip_class(cls);
+  auto mf = [&](int z, int zp) { return f[(size_t)(2 * mf_index(z, zp, sa, sb) + ri) * G + r]; };
+  switch (c) {
+    case 0: return mf(0, 0);
+    case 1: return mf(0, 1);
+    case 2: return mf(1, 0);
+    case 3: return mf(0, 2);
+    case 4: return mf(2, 0);
+    case 5: return mf(0, 3);
+    case 6: return mf(3, 0);
+    case 7: return mf(0, 4);
+    case 8: return flip ? mf(3, 3) : mf(1, 1);
+    case 9: return flip ? mf(1, 1) - mf(3, 3) : mf(3, 3) - mf(1, 1);
+    case 10: return mf(1, 2);
+    case 11: return mf(1, 3);
+    case 12: return mf(3, 1);
+    case 13: return mf(2, 3);
+    case 14: return mf(3, 2);
+    default: return flip ? mf(2, 2) - mf(3, 3) : 0.0;
+  }
+}
+// combinations used per class (K extent of the projection GEMM in tables)
+__host__ __device__ constexpr int proj_ncomb(int cls) { return (cls == 1 || cls == 2) ? 16 : 15;}
 
 // Density tables: per spin class the 28 densities need only nine linear
 // combinations of the products psi_x(r,a) psi_y(r,b) (x, y = psi, d_r, d_phi,
@@ -109,8 +166,7 @@ __global__ void k_mix_tables(TabArgs T) {
   const int c = blockIdx.y;
   int x, y;
   if (T.single) { x = 0; y = 0; }
-  else if (T.dens) { x = -1; y = -1; }       // combinations kDensW
-  else { x = kProjZ[c][0]; y = kProjZ[c][1]; }
+  else { x = -1; y = -1; }                   // combinations kDensW / proj_table
   const size_t n = (size_t)T.Kp * T.RP;
   for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
     int k, r;
@@ -123,13 +179,17 @@ __global__ void k_mix_tables(TabArgs T) {
         v = (float)(T.psi[x][(size_t)a * T.nghl + r] * T.psi[y][(size_t)b * T.nghl + r]);
       } else {
         const int cls = k < T.kbeg[1] ? 0 : k < T.kbeg[2] ? 1 : k < T.kbeg[3] ? 2 : 3;
-        double pa[4], pb[4];
-        for (int q = 0; q < 4; ++q) {
+        double pa[5], pb[5];
+        for (int q = 0; q < 5; ++q) {
           pa[q] = T.psi[q][(size_t)a * T.nghl + r];
           pb[q] = T.psi[q][(size_t)b * T.nghl + r];
         }
         double acc = 0.0;
-        for (int xy = 0; xy < 16; ++xy) acc += (double)kDensW[cls][c][xy] * pa[xy >> 2] * pb[xy & 3];
+        if (T.dens) {
+          for (int xy = 0; xy < 16; ++xy) acc += (double)kDensW[cls][c][xy] * pa[xy >> 2] * pb[xy & 3];
+        } else {
+          acc = proj_table(cls, c, pa, pb);
+        }
         v = (float)acc;
       }
     }
@@ -248,11 +308,9 @@ __global__ void k_mix_fields(CvtFldArgs A) {
   for (int sec = 0; sec < 2; ++sec) {
     const double *f = A.flds + (((size_t)w * 2 + sec) * 2 * NFLD) * A.nghl;
     for (int c = 0; c < NPROJ_C; ++c) {
-      const int z = kProjZ[c][0], zp = kProjZ[c][1];
       for (int cls = 0; cls < 4; ++cls) {
-        const int idx = mf_index(z, zp, cls >> 1, cls & 1);
         for (int ri = 0; ri < 2; ++ri) {
-          const float v = live ? (float)(2.0 * f[(size_t)(2 * idx + ri) * A.nghl + r]) : 0.0f;
+          const float v = live ? (float)(2.0 * proj_field(cls, c, f, A.nghl, r, ri)) : 0.0f;
           __bf16 h, l;
           split_bf16(v, h, l);
           const size_t o = (((size_t)cls * A.ncap + (slot * 2 + sec) * 2 + ri) * NPROJ_C + c) * A.RP + r;

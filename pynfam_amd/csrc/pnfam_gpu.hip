@@ -1599,11 +1599,13 @@ int launch_triprod_bf16(pnfam_gpu_ctx *h, int stage, int ri, const Group *groups
   for (int c = merge ? cmax : 1; c <= TP_MAXCT && c >= 1; ++c) {
     const int n = merge ? ntot : h->tp_count[stage][ri][c];
     if (!n) continue;
-    const dim3 grid(n, na, merge ? smax : h->tp_strips[stage][ri][c]);
+    const int ns = merge ? smax : h->tp_strips[stage][ri][c];
+    const long units8 = ((long)n * na + 7) / 8;
+    const dim3 grid((unsigned)(units8 * ns * 8));
     const Group *g = groups + (merge ? 0 : h->tp_first[stage][ri][c]);
 #define TB_LAUNCH(NC)                                                                                          \
-  hipLaunchKernelGGL(k_triprod_bf16<NC>, grid, dim3(64 * TP_NW), 0, h->stream, g, terms, M, BB, obase, ostride, act, \
-                     h->nxy, n3)
+  hipLaunchKernelGGL(k_triprod_bf16<NC>, grid, dim3(64 * TP_NW), 0, h->stream, g, n, ns, terms, M, BB, obase, ostride, \
+                     act, na, h->nxy, n3)
     switch (c) {
       case 1: TB_LAUNCH(1); break;
       case 2: TB_LAUNCH(2); break;
@@ -2081,7 +2083,9 @@ int stage_hamiltonian_mixed(pnfam_gpu_ctx *h, int na, const int *act, int n3) {
         const int64_t k0 = S.kbeg[c], kl = S.kbeg[c + 1] - S.kbeg[c];
         if (kl == 0) continue;
         const size_t fo = (size_t)c * ncol * KK;
-        if (int e = blas_split(h, nS, kl, KK, S.Fh.as<__bf16>() + fo, S.Fl.as<__bf16>() + fo, KK,
+        // the mean-field set uses proj_ncomb(c) of the NPROJ_C table combinations
+        const int64_t Kc = si == 0 ? (int64_t)proj_ncomb(c) * RP : KK;
+        if (int e = blas_split(h, nS, kl, Kc, S.Fh.as<__bf16>() + fo, S.Fl.as<__bf16>() + fo, KK,
                                S.TPh.as<__bf16>() + (size_t)k0 * KK, S.TPl.as<__bf16>() + (size_t)k0 * KK, KK,
                                O + (size_t)k0 * ncol, ncol, (si == 0 ? 4 : 2) * (int64_t)n3))
           return e;
