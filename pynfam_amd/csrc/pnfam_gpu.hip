@@ -1365,8 +1365,16 @@ struct pnfam_gpu_ctx {
   std::vector<int4> tp_blocks;  // (fp64 offset, padded offset, d, P) per block
   long tp_mstride = 0;
   DevPtr tp_rmh, tp_rml, tp_cmh, tp_cml, d_tp_blocks;
-  DevPtr d_tp_toff[2][2], d_tp_jobs[2][2], tp_Bh, tp_Bl;   // [stage][ri]: B operand layout and conversion jobs
-  int tp_njobs[2][2] = {{0, 0}, {0, 0}};
+  // per stage, the re and im term sets together (one launch per tile class
+  // covers both): terms, groups sorted by tile count, B operand offsets and
+  // conversion jobs
+  std::vector<Term> tpa_terms_h[2];
+  std::vector<Group> tpa_groups_h[2];
+  std::vector<int> tpa_toff_h[2];
+  std::vector<int2> tpa_jobs_h[2];
+  DevPtr d_tpa_terms[2], d_tpa_groups[2], d_tpa_toff[2], d_tpa_jobs[2], tp_Bh, tp_Bl;
+  int tpa_njobs[2] = {0, 0};
+  int tpa_first[2][TP_MAXCT + 1] = {}, tpa_count[2][TP_MAXCT + 1] = {}, tpa_strips[2][TP_MAXCT + 1] = {};
   long tp_L[2] = {0, 0}, tp_Lcap = 0;                        // per active point, per stage
   int unit_precision = PNFAM_GPU_FP64;   // grid stage of the per-unit entry points
   int bro_nwg = 0;
@@ -1521,11 +1529,12 @@ void tp_patch_terms(const pnfam_gpu_ctx *h, std::vector<Term> &terms) {
   for (auto &t : terms) { t.pad0 = pad(t.a_off); t.pad1 = pad(t.c_off); }
 }
 
-// B operand layout of one (stage, ri): term offsets (after base) and
-// conversion jobs; returns the end offset.
-long tp_b_layout(pnfam_gpu_ctx *h, int stage, int ri, const std::vector<Term> &terms, long base, int &err) {
-  std::vector<int> toff(std::max<size_t>(1, terms.size()), 0);
-  std::vector<int2> jobs;
+// B operand layout of one (stage, ri) term set, appended to the stage's
+// combined lists: term offsets (after base) and conversion jobs; returns the
+// end offset.
+long tp_b_layout(pnfam_gpu_ctx *h, int stage, const std::vector<Term> &terms, const std::vector<Group> &groups,
+                 long base) {
+  const int t0 = (int)h->tpa_terms_h[stage].size();
   long off = base;
   // terms that read the same block of B the same way share one converted copy
   std::map<std::array<int, 5>, int> seen;
@@ -1533,18 +1542,50 @@ long tp_b_layout(pnfam_gpu_ctx *h, int stage, int ri, const std::vector<Term> &t
     const Term &t = terms[i];
     const std::array<int, 5> key{t.b_slot, t.b_off, t.tb, t.nda, t.ndc};
     const auto it = seen.find(key);
-    if (it != seen.end()) { toff[i] = it->second; continue; }
+    if (it != seen.end()) { h->tpa_toff_h[stage].push_back(it->second); continue; }
     const int nj = (t.ndc + 15) / 16, nk = ((t.nda + 31) & ~31) / TP_TK;
-    toff[i] = (int)off;
+    h->tpa_toff_h[stage].push_back((int)off);
     seen[key] = (int)off;
-    for (int q = 0; q < nj * nk; ++q) jobs.push_back(make_int2((int)i, q));
+    for (int q = 0; q < nj * nk; ++q) h->tpa_jobs_h[stage].push_back(make_int2(t0 + (int)i, q));
     off += (long)nj * TP_TJ * nk * TP_TK;
   }
-  if (jobs.empty()) jobs.push_back(make_int2(0, 0));
-  h->tp_njobs[stage][ri] = terms.empty() ? 0 : (int)jobs.size();
-  err = upload(h->d_tp_toff[stage][ri], toff.data(), toff.size() * 4, h->stream);
-  if (!err) err = upload(h->d_tp_jobs[stage][ri], jobs.data(), jobs.size() * sizeof(int2), h->stream);
+  h->tpa_terms_h[stage].insert(h->tpa_terms_h[stage].end(), terms.begin(), terms.end());
+  for (Group g : groups) {
+    g.first += t0;
+    h->tpa_groups_h[stage].push_back(g);
+  }
   return off;
+}
+
+// Sorts a stage's combined groups by tile count and uploads the combined lists.
+int tp_finish_stage(pnfam_gpu_ctx *h, int stage) {
+  auto &groups = h->tpa_groups_h[stage];
+  auto nct = [](const Group &g) { return (g.ncol + 15) / 16; };
+  std::stable_sort(groups.begin(), groups.end(), [&](const Group &a, const Group &b) { return nct(a) < nct(b); });
+  for (int c = 0; c <= TP_MAXCT; ++c) { h->tpa_first[stage][c] = 0; h->tpa_count[stage][c] = 0; h->tpa_strips[stage][c] = 0; }
+  for (size_t i = 0; i < groups.size(); ++i) {
+    const int c = nct(groups[i]);
+    if (h->tpa_count[stage][c] == 0) h->tpa_first[stage][c] = (int)i;
+    h->tpa_count[stage][c]++;
+    h->tpa_strips[stage][c] = std::max(h->tpa_strips[stage][c], (groups[i].nrow + 15) / 16);
+  }
+  auto &terms = h->tpa_terms_h[stage];
+  auto &toff = h->tpa_toff_h[stage];
+  auto &jobs = h->tpa_jobs_h[stage];
+  h->tpa_njobs[stage] = (int)jobs.size();
+  if (terms.empty()) terms.push_back(Term{});
+  if (toff.empty()) toff.push_back(0);
+  if (jobs.empty()) jobs.push_back(make_int2(0, 0));
+  if (groups.empty()) groups.push_back(Group{});
+  int e;
+  if ((e = upload(h->d_tpa_terms[stage], terms.data(), terms.size() * sizeof(Term), h->stream)) ||
+      (e = upload(h->d_tpa_groups[stage], groups.data(), groups.size() * sizeof(Group), h->stream)) ||
+      (e = upload(h->d_tpa_toff[stage], toff.data(), toff.size() * 4, h->stream)) ||
+      (e = upload(h->d_tpa_jobs[stage], jobs.data(), jobs.size() * sizeof(int2), h->stream)))
+    return e;
+  HIP_TRY(hipStreamSynchronize(h->stream));   // the host lists are released below
+  terms.clear(); toff.clear(); jobs.clear(); groups.clear();
+  return 0;
 }
 
 bool tp_terms_ok(const std::vector<Term> &terms) {
@@ -1572,9 +1613,10 @@ int tp_prepare(pnfam_gpu_ctx *h) {
   return 0;
 }
 
-int launch_triprod_bf16(pnfam_gpu_ctx *h, int stage, int ri, const Group *groups, const Term *terms,
-                        const double *bbase, long bstride, double *obase, long ostride, int na, const int *act,
-                        int n3) {
+// Both term sets (re and im) of a stage: one conversion launch and one launch
+// per tile class (one launch in all for few points).
+int launch_triprod_bf16(pnfam_gpu_ctx *h, int stage, const double *bbase, long bstride, double *obase, long ostride,
+                        int na, const int *act, int n3) {
   if (int e = tp_prepare(h)) return e;
   const long L = std::max(h->tp_L[0], h->tp_L[1]);
   if (h->tp_Lcap < L * h->cap) {
@@ -1583,26 +1625,28 @@ int launch_triprod_bf16(pnfam_gpu_ctx *h, int stage, int ri, const Group *groups
     h->tp_Lcap = L * h->cap;
   }
   if (na > h->cap) return fail(PNFAM_GPU_EINVAL, "more active points than the batch capacity");
-  if (h->tp_njobs[stage][ri])
-    hipLaunchKernelGGL(k_tp_bconv, dim3(h->tp_njobs[stage][ri], na), dim3(256), 0, h->stream, terms,
-                       h->d_tp_toff[stage][ri].as<int>(), h->d_tp_jobs[stage][ri].as<int2>(), bbase, bstride, h->nxy,
+  const Term *terms = h->d_tpa_terms[stage].as<Term>();
+  const Group *groups = h->d_tpa_groups[stage].as<Group>();
+  if (h->tpa_njobs[stage])
+    hipLaunchKernelGGL(k_tp_bconv, dim3(h->tpa_njobs[stage], na), dim3(256), 0, h->stream, terms,
+                       h->d_tpa_toff[stage].as<int>(), h->d_tpa_jobs[stage].as<int2>(), bbase, bstride, h->nxy,
                        act, L, h->tp_Bh.as<__bf16>(), h->tp_Bl.as<__bf16>());
   TpMats M{h->tp_rmh.as<__bf16>(), h->tp_rml.as<__bf16>(), h->tp_cmh.as<__bf16>(), h->tp_cml.as<__bf16>(),
            h->tp_mstride};
-  TpB BB{h->tp_Bh.as<__bf16>(), h->tp_Bl.as<__bf16>(), L, h->d_tp_toff[stage][ri].as<int>()};
+  TpB BB{h->tp_Bh.as<__bf16>(), h->tp_Bl.as<__bf16>(), L, h->d_tpa_toff[stage].as<int>()};
   // few points: one launch over all tile classes (the widest template), so the
   // classes' dependency chains overlap instead of running one after another
   const bool merge = na <= TP_MERGE_N;
   int cmax = 0, smax = 0, ntot = 0;
   for (int c = 1; c <= TP_MAXCT; ++c)
-    if (h->tp_count[stage][ri][c]) { cmax = c; smax = std::max(smax, h->tp_strips[stage][ri][c]); ntot += h->tp_count[stage][ri][c]; }
+    if (h->tpa_count[stage][c]) { cmax = c; smax = std::max(smax, h->tpa_strips[stage][c]); ntot += h->tpa_count[stage][c]; }
   for (int c = merge ? cmax : 1; c <= TP_MAXCT && c >= 1; ++c) {
-    const int n = merge ? ntot : h->tp_count[stage][ri][c];
+    const int n = merge ? ntot : h->tpa_count[stage][c];
     if (!n) continue;
-    const int ns = merge ? smax : h->tp_strips[stage][ri][c];
+    const int ns = merge ? smax : h->tpa_strips[stage][c];
     const long units8 = ((long)n * na + 7) / 8;
     const dim3 grid((unsigned)(units8 * ns * 8));
-    const Group *g = groups + (merge ? 0 : h->tp_first[stage][ri][c]);
+    const Group *g = groups + (merge ? 0 : h->tpa_first[stage][c]);
 #define TB_LAUNCH(NC)                                                                                          \
   hipLaunchKernelGGL(k_triprod_bf16<NC>, grid, dim3(64 * TP_NW), 0, h->stream, g, n, ns, terms, M, BB, obase, ostride, \
                      act, na, h->nxy, n3)
@@ -1672,14 +1716,9 @@ int stage_qp2sp(pnfam_gpu_ctx *h, const double *vin, int na, const int *act, boo
   { Prof pr(h, K_ZERO, na);
     hipLaunchKernelGGL(k_zero_slots, dim3(64, na), dim3(256), 0, h->stream, h->d_rsp.as<double>(), act, 8, 0, 8, nxy); }
   Prof pr(h, K_TRIPROD_Q2S, na);
+  if (lowp) return launch_triprod_bf16(h, 0, vin, 4 * nxy, h->d_rsp.as<double>(), 8 * nxy, na, act, n3);
   for (int ri = 0; ri < 2; ++ri) {
     if (!h->ngroups_q2s[ri]) continue;
-    if (lowp) {
-      if (int e = launch_triprod_bf16(h, 0, ri, h->d_groups_q2s[ri].as<Group>(), h->d_terms_q2s[ri].as<Term>(), vin,
-                                      4 * nxy, h->d_rsp.as<double>(), 8 * nxy, na, act, n3))
-        return e;
-      continue;
-    }
     if (!h->basic_kernels) {
       if (int e = launch_triprod(h, 0, ri, h->d_groups_q2s[ri].as<Group>(), h->d_terms_q2s[ri].as<Term>(), vin, 4 * nxy,
                                  h->d_rsp.as<double>(), 8 * nxy, na, act))
@@ -2324,14 +2363,9 @@ int stage_sp2qp(pnfam_gpu_ctx *h, int na, const int *act, bool lowp = false, int
   { Prof pr(h, K_ZERO, na);
     hipLaunchKernelGGL(k_zero_slots, dim3(64, na), dim3(256), 0, h->stream, h->d_hqp.as<double>(), act, 4, 0, 4, nxy); }
   Prof pr(h, K_TRIPROD_S2Q, na);
+  if (lowp) return launch_triprod_bf16(h, 1, h->d_hsp.as<double>(), 8 * nxy, h->d_hqp.as<double>(), 4 * nxy, na, act, n3);
   for (int ri = 0; ri < 2; ++ri) {
     if (!h->ngroups_s2q[ri]) continue;
-    if (lowp) {
-      if (int e = launch_triprod_bf16(h, 1, ri, h->d_groups_s2q[ri].as<Group>(), h->d_terms_s2q[ri].as<Term>(),
-                                      h->d_hsp.as<double>(), 8 * nxy, h->d_hqp.as<double>(), 4 * nxy, na, act, n3))
-        return e;
-      continue;
-    }
     if (!h->basic_kernels) {
       if (int e = launch_triprod(h, 1, ri, h->d_groups_s2q[ri].as<Group>(), h->d_terms_s2q[ri].as<Term>(),
                                  h->d_hsp.as<double>(), 8 * nxy, h->d_hqp.as<double>(), 4 * nxy, na, act))
@@ -2495,8 +2529,9 @@ int pnfam_gpu_set_operator(pnfam_gpu_ctx *h, int32_t beta_minus, const int32_t *
     tp_patch_terms(h, terms);
     if (!tp_terms_ok(terms)) h->tp_bf16 = false;   // A or C not whole blocks: FP64 products only
     int e;
-    h->tp_L[0] = tp_b_layout(h, 0, ri, terms, ri == 0 ? 0 : h->tp_L[0], e);
-    if (e) return e;
+    if (ri == 0) { h->tpa_terms_h[0].clear(); h->tpa_groups_h[0].clear(); h->tpa_toff_h[0].clear(); h->tpa_jobs_h[0].clear(); }
+    h->tp_L[0] = tp_b_layout(h, 0, terms, groups, ri == 0 ? 0 : h->tp_L[0]);
+    if (ri == 1 && (e = tp_finish_stage(h, 0))) return e;
     if ((e = upload(h->d_terms_q2s[ri], terms.data(), std::max<size_t>(1, terms.size()) * sizeof(Term), h->stream)) ||
         (e = upload(h->d_groups_q2s[ri], groups.data(), std::max<size_t>(1, groups.size()) * sizeof(Group), h->stream)))
       return e;
@@ -2534,8 +2569,9 @@ int pnfam_gpu_set_operator(pnfam_gpu_ctx *h, int32_t beta_minus, const int32_t *
     tp_patch_terms(h, terms);
     if (!tp_terms_ok(terms)) h->tp_bf16 = false;   // A or C not whole blocks: FP64 products only
     int e;
-    h->tp_L[1] = tp_b_layout(h, 1, ri, terms, ri == 0 ? 0 : h->tp_L[1], e);
-    if (e) return e;
+    if (ri == 0) { h->tpa_terms_h[1].clear(); h->tpa_groups_h[1].clear(); h->tpa_toff_h[1].clear(); h->tpa_jobs_h[1].clear(); }
+    h->tp_L[1] = tp_b_layout(h, 1, terms, groups, ri == 0 ? 0 : h->tp_L[1]);
+    if (ri == 1 && (e = tp_finish_stage(h, 1))) return e;
     if ((e = upload(h->d_terms_s2q[ri], terms.data(), std::max<size_t>(1, terms.size()) * sizeof(Term), h->stream)) ||
         (e = upload(h->d_groups_s2q[ri], groups.data(), std::max<size_t>(1, groups.size()) * sizeof(Group), h->stream)))
       return e;
