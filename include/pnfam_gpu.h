@@ -73,7 +73,7 @@ typedef struct {
  *   MIXED_NOREFINE: as MIXED without the final FP64 evaluation; S then carries
  *     the low-precision error of the first increments (about 1e-6 relative).
  *   In both mixed modes a point whose increment, judged by its last residual,
- *   is below 1e-3 of its largest amplitude (environment PNFAM_GPU_MIX_TAU)
+ *   is below 3e-3 of its largest amplitude (environment PNFAM_GPU_MIX_TAU)
  *   goes through single BF16 products instead of the three split products:
  *   its rounding error is scaled down by the increment.  The grid stage runs as
  *   GEMMs against resident pair tables or as factored kernels whose cost is per

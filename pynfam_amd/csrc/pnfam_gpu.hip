@@ -2876,7 +2876,7 @@ int pnfam_gpu_solve_batch(pnfam_gpu_ctx *h, int32_t npts, const double *re_w, co
   const double *e2 = h->bminus ? h->d_En.as<double>() : h->d_Ep.as<double>();
   std::vector<double> si(npts, 1.0), S(2 * npts, 0.0), vmax(npts, 1.0);
   std::vector<int> done(npts, 0), refined(npts, 0), act, act64, actlo;
-  double tau = 1e-3;   // one-product threshold of the mixed path
+  double tau = 3e-3;   // one-product threshold of the mixed path
   if (const char *ev = std::getenv("PNFAM_GPU_MIX_TAU")) tau = std::atof(ev);
   for (int w = 0; w < npts; ++w) { iters[w] = -1; re_s[w] = 0.0; im_s[w] = 0.0; }
   const int nwg = h->bro_nwg;

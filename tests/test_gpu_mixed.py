@@ -17,7 +17,7 @@ pytestmark = pytest.mark.gpu
 TOL = {"MIXED": 1e-6, "MIXED_NOREFINE": 1e-4}
 
 
-@pytest.mark.parametrize("grid", ["tables", "factored"])
+@pytest.mark.parametrize("grid", ["tables", "factored", "auto"])
 @pytest.mark.parametrize("mode", ["MIXED", "MIXED_NOREFINE"])
 @pytest.mark.parametrize("nucleus,K", [("S40", 0), ("S40", 1), ("Gd162", 0), ("Gd162", 1)])
 def test_mixed_matches_reference_runs(nucleus, K, mode, grid, monkeypatch):
