@@ -30,11 +30,12 @@ struct FacBlock {             // chunk bases and counts of the two spins of a bl
 };
 
 // ---- omega-independent tables --------------------------------------------
-// PA[z][state][RP]: psi_z(r, state), zero for r >= nghl.
+// PA[z][state][RP]: psi_z(r, state), zero for r >= nghl; PS the same in FP32
+// (the operand of the vector work: B of the projection, psi_y of the densities).
 __global__ void k_fac_ptab(const double *__restrict__ psi0, const double *__restrict__ psi1,
                            const double *__restrict__ psi2, const double *__restrict__ psi3,
                            const double *__restrict__ psi4, int nst, int nghl, int RP, __bf16 *__restrict__ hi,
-                           __bf16 *__restrict__ lo) {
+                           __bf16 *__restrict__ lo, float *__restrict__ ps) {
   const int z = blockIdx.y;
   const double *p = z == 0 ? psi0 : z == 1 ? psi1 : z == 2 ? psi2 : z == 3 ? psi3 : psi4;
   const size_t n = (size_t)nst * RP;
@@ -46,6 +47,7 @@ __global__ void k_fac_ptab(const double *__restrict__ psi0, const double *__rest
     split_bf16(v, h, l);
     hi[(size_t)z * n + e] = h;
     lo[(size_t)z * n + e] = l;
+    ps[(size_t)z * n + e] = v;
   }
 }
 
@@ -113,7 +115,9 @@ struct FacDensArgs {
   const FacBlock *blk;
   const int4 *pairs;              // (ix, iy, FR offset, 0) per pair
   int rho_first[2], rho_n[2], kap_first[2], kap_n[2];   // pair ranges per sector
-  const double *wf, *wfdr, *wfdp, *wfdz, *wdcori;
+  const float *PS;                // [z][state][RP] FP32: psi_y of the products (z = 0..3)
+  long pz;                        // elements per z plane of PS
+  const double *wdcori;
   const int *isstart, *db, *nsu;
   int nghl, RP;
   double *dens;                   // [w][2][56][nghl]
@@ -156,7 +160,6 @@ __global__ void __launch_bounds__(64 * FAC_NW, 2) k_fac_density(FacDensArgs A) {
   const int G = A.nghl;
   const int r = r0 + fi;
   const bool rok = r < G;
-  const int rc = rok ? r : 0;
   const bool three = slot < A.n3;
   const int pstep = FAC_NW * A.psplit, poff = wave + FAC_NW * prt;
   float acc[2 * NDENS];
@@ -164,7 +167,6 @@ __global__ void __launch_bounds__(64 * FAC_NW, 2) k_fac_density(FacDensArgs A) {
   for (int d = 0; d < 2 * NDENS; ++d) acc[d] = 0.0f;
   const __bf16 *frh[2] = {A.FRh + ((size_t)slot * 2 + 0) * A.L, A.FRh + ((size_t)slot * 2 + 1) * A.L};
   const __bf16 *frl[2] = {A.FRl + ((size_t)slot * 2 + 0) * A.L, A.FRl + ((size_t)slot * 2 + 1) * A.L};
-  const double *tab[4] = {A.wf, A.wfdr, A.wfdp, A.wfdz};
   const v4f z4 = {0.f, 0.f, 0.f, 0.f};
 
   // mean-field pairs: x = 0..3
@@ -208,10 +210,10 @@ __global__ void __launch_bounds__(64 * FAC_NW, 2) k_fac_density(FacDensArgs A) {
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
           if (b0 + 4 * fg + v >= bend || !rok) break;
-          const size_t oy = (size_t)(ib0 + b0 + 4 * fg + v) * G + rc;
+          const size_t oy = (size_t)(ib0 + b0 + 4 * fg + v) * A.RP + r;
           float y[4];
 #pragma unroll
-          for (int yy = 0; yy < 4; ++yy) y[yy] = (float)tab[yy][oy];
+          for (int yy = 0; yy < 4; ++yy) y[yy] = A.PS[(size_t)yy * A.pz + oy];
           float Pr[4][4], Pi[4][4], Qr[4][4], Qi[4][4];
 #pragma unroll
           for (int x = 0; x < 4; ++x)
@@ -238,7 +240,7 @@ __global__ void __launch_bounds__(64 * FAC_NW, 2) k_fac_density(FacDensArgs A) {
       for (int b0 = bbeg; b0 < bend; b0 += 16) {
         float wbv[4];
 #pragma unroll
-        for (int v = 0; v < 4; ++v) wbv[v] = (float)A.wf[(size_t)(ib0 + min(b0 + 4 * fg + v, bend - 1)) * G + rc];
+        for (int v = 0; v < 4; ++v) wbv[v] = A.PS[(size_t)(ib0 + min(b0 + 4 * fg + v, bend - 1)) * A.RP + r];
         v4f T[2][2];
 #pragma unroll
         for (int sa = 0; sa < 2; ++sa) { T[sa][0] = z4; T[sa][1] = z4; }
@@ -314,10 +316,10 @@ __global__ void k_fac_dsum(const float *__restrict__ part, int psplit, int na, i
 // ---- projection -------------------------------------------------------------
 struct FacProjArgs {
   const __bf16 *PAh, *PAl;        // [z][state][RP]
-  long pz;                        // elements per z plane of PA
-  const double *psi[5];           // FP64 [state][nghl]
+  const float *PS;                // [z][state][RP] FP32
+  long pz;                        // elements per z plane of PA and PS
   const int *isstart, *db, *nsu;
-  const double *flds;             // [w][2][208][nghl]
+  const float *FFm, *FFp;         // staged fields of the active slots (k_fac_fconv)
   double *hsp;                    // [w][8][nxy]
   long nxy;
   int nghl, RP;
@@ -331,6 +333,40 @@ struct FacProjArgs {
 __host__ __device__ constexpr int fac_zp_count(int z) { return z == 0 ? 5 : (z == 4 ? 1 : 4); }
 __host__ __device__ constexpr int fac_zp_first(int z) { return z == 0 ? 0 : (z == 4 ? 17 : 5 + 4 * (z - 1)); }
 
+constexpr int FAC_NMF = 18;                 // non-zero (z, z') mean-field combos
+constexpr int FAC_FM = 2 * FAC_NMF * 2;     // FFm floats per grid point: [s_b][combo][re, im]
+constexpr int FAC_FP = 2 * 2;               // FFp floats per grid point: [s_b][re, im]
+
+// The fields of the active slots in the order the projection stages them, per
+// (slot, sector, spin of a), zero for r >= nghl:
+//   FFm[slot][sec][s_a][RP][s_b][combo][ri]   mf(z, z', s_a, s_b), combos as fac_zp_first
+//   FFp[slot][sec][s_a][RP][s_b][ri]          pairing field of (s_a, s_b)
+__global__ void __launch_bounds__(256) k_fac_fconv(const double *__restrict__ flds, int nghl, int RP,
+                                                   const int *__restrict__ act, float *__restrict__ ffm,
+                                                   float *__restrict__ ffp) {
+  const int slot = blockIdx.y, sec = blockIdx.z >> 1, sa = blockIdx.z & 1;
+  const int w = act[slot];
+  const double *fl = flds + (((size_t)w * 2 + sec) * 2 * NFLD) * nghl;
+  const size_t plane = ((size_t)slot * 2 + sec) * 2 + sa;
+  float *om = ffm + plane * RP * FAC_FM;
+  float *op = ffp + plane * RP * FAC_FP;
+  const int nm = RP * FAC_FM, n = nm + RP * FAC_FP;
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+    if (e < nm) {
+      const int ri = e & 1, cc = (e >> 1) % FAC_NMF, sb = (e / (2 * FAC_NMF)) & 1, r = e / FAC_FM;
+      int z = 0, zp = cc;
+      if (cc >= 17) { z = 4; zp = 0; }
+      else if (cc >= 5) { z = 1 + (cc - 5) / 4; zp = (cc - 5) % 4; }
+      om[e] = r < nghl ? (float)fl[(size_t)(2 * mf_index(z, zp, sa, sb) + ri) * nghl + r] : 0.0f;
+    } else {
+      const int e2 = e - nm;
+      const int ri = e2 & 1, sb = (e2 >> 1) & 1, r = e2 >> 2;
+      const int f = 100 + (sb == 0 ? (sa == 0 ? 0 : 1) : (sa == 0 ? 2 : 3));
+      op[e2] = r < nghl ? (float)fl[(size_t)(2 * f + ri) * nghl + r] : 0.0f;
+    }
+  }
+}
+
 // One workgroup of FAC_NW waves per (projection tile, frequency): rows NRT x 16
 // states of spin s_a of block ix, 16 columns of block iy, K = (z, r).  Wave j
 // takes the 32-point steps j, j + FAC_NW, ...; each wave stages the fields of
@@ -341,13 +377,10 @@ __global__ void __launch_bounds__(64 * FAC_NW, 2) k_fac_project(FacProjArgs P, c
   constexpr int NZ = KIND == 0 ? 5 : 1;
   constexpr int NC = KIND == 0 ? 18 : 1;
   constexpr int STG = FAC_RT * 2 * NC * 2;            // staged floats per wave and step
-  constexpr int NPL = (STG + 63) / 64;                // per lane
   constexpr int RED = (FAC_NW - 1) * NRT * 8 * 64;    // reduction floats
   constexpr int SM = FAC_NW * STG > RED ? FAC_NW * STG : RED;
   __shared__ __attribute__((aligned(16))) float smem[SM];
   const Tile2 t = tiles[blockIdx.x];
-  const int w = P.act[blockIdx.y];
-  const int G = P.nghl;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int fi = lane & 15, fg = lane >> 4;
   float *sMF = smem + wave * STG;                      // [r][s_b][combo][ri]
@@ -355,45 +388,32 @@ __global__ void __launch_bounds__(64 * FAC_NW, 2) k_fac_project(FacProjArgs P, c
   const int dx = P.db[t.ix], dy = P.db[t.iy];
   const int nrows = t.a_hi - t.a_lo;
   const bool three = (int)blockIdx.y < P.n3;
-  const double *fl = P.flds + (((size_t)w * 2 + t.sec) * 2 * NFLD) * G;
+  // this tile's staged fields: STG consecutive floats per 32-point step
+  const float *ff = (KIND == 0 ? P.FFm : P.FFp) + (((size_t)blockIdx.y * 2 + t.sec) * 2 + t.sa) * P.RP * (STG / FAC_RT);
   const int bj = min(t.b0 + fi, dy - 1);           // this lane's column (clamped; outputs masked)
   const int sb = bj < P.nsu[t.iy] ? 0 : 1;
-  const double *pcol[NZ];
-#pragma unroll
-  for (int z = 0; z < NZ; ++z) pcol[z] = P.psi[z] + (size_t)(ib0 + bj) * G;
+  const float *pcol = P.PS + (size_t)(ib0 + bj) * P.RP;
   int arow[NRT];
 #pragma unroll
   for (int q = 0; q < NRT; ++q) arow[q] = ia + t.a_lo + min(q * 16 + fi, nrows - 1);
-  // staged value k of this lane: grid point rr, re/im ri, combo k % NC, spin of b k / NC
-  static_assert(STG % 64 == 0, "whole staging rounds");
-  const int rr = lane & (FAC_RT - 1), ri_l = (lane >> 5) & 1;
-  auto field_of = [&](int k) {      // k is a compile-time constant after unrolling
-    const int cc = k % NC, sbb = k / NC;
-    if (KIND == 0) {
-      int z = 0, zp = cc;
-      if (cc >= 17) { z = 4; zp = 0; }
-      else if (cc >= 5) { z = 1 + (cc - 5) / 4; zp = (cc - 5) % 4; }
-      return mf_index(z, zp, t.sa, sbb);
-    }
-    return 100 + (sbb == 0 ? (t.sa == 0 ? 0 : 1) : (t.sa == 0 ? 2 : 3));
-  };
-  const double *flane = fl + (size_t)ri_l * G + rr;
-  auto fetch = [&](int rc, float (&v)[NPL]) {
-    const bool ok = rc + rr < G;
-#pragma unroll
-    for (int k = 0; k < NPL; ++k) v[k] = ok ? (float)flane[(size_t)(2 * field_of(k)) * G + rc] : 0.0f;
-  };
+  static_assert(STG % 128 == 0, "whole staging rounds of float2");
   v4f C[NRT][2];
   const v4f z4 = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
   for (int q = 0; q < NRT; ++q) { C[q][0] = z4; C[q][1] = z4; }
   for (int rc = wave * FAC_RT; rc < P.RP; rc += FAC_NW * FAC_RT) {
-    // this step's fields into the wave's slice
+    // this step's fields into the wave's slice (a straight copy)
     {
-      float cur[NPL];
-      fetch(rc, cur);
+      const float *src = ff + (size_t)rc * (STG / FAC_RT);
+      if constexpr (STG % 256 == 0) {
 #pragma unroll
-      for (int k = 0; k < NPL; ++k) sMF[((rr * 2 + k / NC) * NC + k % NC) * 2 + ri_l] = cur[k];
+        for (int k = 0; k < STG / 256; ++k)
+          ((float4 *)sMF)[lane + 64 * k] = ((const float4 *)src)[lane + 64 * k];
+      } else {
+#pragma unroll
+        for (int k = 0; k < STG / 128; ++k)
+          ((float2 *)sMF)[lane + 64 * k] = ((const float2 *)src)[lane + 64 * k];
+      }
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -401,9 +421,12 @@ __global__ void __launch_bounds__(64 * FAC_NW, 2) k_fac_project(FacProjArgs P, c
     const int rb = rc + 8 * fg;
     float pz[NZ][8];
 #pragma unroll
-    for (int z = 0; z < NZ; ++z)
-#pragma unroll
-      for (int i = 0; i < 8; ++i) pz[z][i] = (float)pcol[z][min(rb + i, G - 1)];
+    for (int z = 0; z < NZ; ++z) {
+      const float4 *p4 = (const float4 *)(pcol + (size_t)z * P.pz + rb);
+      const float4 u = p4[0], v = p4[1];
+      pz[z][0] = u.x; pz[z][1] = u.y; pz[z][2] = u.z; pz[z][3] = u.w;
+      pz[z][4] = v.x; pz[z][5] = v.y; pz[z][6] = v.z; pz[z][7] = v.w;
+    }
     // per z: B_z for this lane's column and 8 grid points, split, then the
     // products with every row tile
 #pragma unroll
@@ -460,6 +483,7 @@ __global__ void __launch_bounds__(64 * FAC_NW, 2) k_fac_project(FacProjArgs P, c
 #pragma unroll
         for (int v = 0; v < 4; ++v) C[q][ri][v] += red[((q * 2 + ri) * 4 + v) * 64 + lane];
   }
+  const int w = P.act[blockIdx.y];
   double *ore = P.hsp + ((size_t)w * 8 + 2 * t.out_slot) * P.nxy + t.off;
   double *oim = ore + P.nxy;
   const int bcol = t.b0 + fi;

@@ -1355,7 +1355,7 @@ struct pnfam_gpu_ctx {
   int mix_grid = 0;             // 0 auto, 1 pair tables, 2 factored (PNFAM_GPU_MIX_GRID)
   int mix_auto_n = 6;           // auto: factored at or below this many active points
   bool fac_ready = false;
-  DevPtr PAh, PAl, DBh, DBl, d_fac_blk, d_fac_pairs, d_fac_segs, FRh, FRl, d_fac_dpart;
+  DevPtr PAh, PAl, PS32, DBh, DBl, d_fac_blk, d_fac_pairs, d_fac_segs, FRh, FRl, d_fac_dpart, FFm, FFp;
   long fac_L = 0, fac_pz = 0, fac_dbx = 0;
   int fac_nseg = 0, fac_cap = 0;
   int fac_first[4] = {0, 0, 0, 0}, fac_n[4] = {0, 0, 0, 0};   // rho0, kap0, rho1, kap1 pair ranges
@@ -2218,11 +2218,12 @@ int fac_prepare(pnfam_gpu_ctx *h) {
   h->fac_pz = (long)nst * RP;
   h->fac_dbx = (long)nchunk * RP * FAC_CH;
   if ((e = h->PAh.alloc(5 * h->fac_pz * 2)) || (e = h->PAl.alloc(5 * h->fac_pz * 2)) ||
-      (e = h->DBh.alloc(4 * h->fac_dbx * 2)) || (e = h->DBl.alloc(4 * h->fac_dbx * 2)))
+      (e = h->PS32.alloc(5 * h->fac_pz * 4)) || (e = h->DBh.alloc(4 * h->fac_dbx * 2)) ||
+      (e = h->DBl.alloc(4 * h->fac_dbx * 2)))
     return e;
   hipLaunchKernelGGL(k_fac_ptab, dim3(2048, 5), dim3(256), 0, h->stream, h->d_wf.as<double>(), h->d_wfdr.as<double>(),
                      h->d_wfdp.as<double>(), h->d_wfdz.as<double>(), h->d_wfd2.as<double>(), nst, h->nghl, RP,
-                     h->PAh.as<__bf16>(), h->PAl.as<__bf16>());
+                     h->PAh.as<__bf16>(), h->PAl.as<__bf16>(), h->PS32.as<float>());
   hipLaunchKernelGGL(k_fac_dtab, dim3(2048, 4), dim3(256), 0, h->stream, h->d_wf.as<double>(), h->d_wfdr.as<double>(),
                      h->d_wfdp.as<double>(), h->d_wfdz.as<double>(), d_chunks.as<int2>(), nchunk, h->nghl, RP,
                      h->DBh.as<__bf16>(), h->DBl.as<__bf16>());
@@ -2260,6 +2261,8 @@ int fac_workspace(pnfam_gpu_ctx *h, int cap) {
   const size_t n = (size_t)cap * 2 * h->fac_L * 2;
   if (int e = h->FRh.alloc(n)) return e;
   if (int e = h->FRl.alloc(n)) return e;
+  if (int e = h->FFm.alloc((size_t)cap * 4 * h->RP * FAC_FM * 4)) return e;
+  if (int e = h->FFp.alloc((size_t)cap * 4 * h->RP * FAC_FP * 4)) return e;
   h->fac_cap = cap;
   return 0;
 }
@@ -2278,8 +2281,7 @@ int stage_hamiltonian_fac(pnfam_gpu_ctx *h, int na, const int *act, int n3) {
   D.blk = h->d_fac_blk.as<FacBlock>(); D.pairs = h->d_fac_pairs.as<int4>();
   D.rho_first[0] = h->fac_first[0]; D.rho_n[0] = h->fac_n[0]; D.kap_first[0] = h->fac_first[1]; D.kap_n[0] = h->fac_n[1];
   D.rho_first[1] = h->fac_first[2]; D.rho_n[1] = h->fac_n[2]; D.kap_first[1] = h->fac_first[3]; D.kap_n[1] = h->fac_n[3];
-  D.wf = h->d_wf.as<double>(); D.wfdr = h->d_wfdr.as<double>(); D.wfdp = h->d_wfdp.as<double>();
-  D.wfdz = h->d_wfdz.as<double>(); D.wdcori = h->d_wdcori.as<double>();
+  D.PS = h->PS32.as<float>(); D.pz = h->fac_pz; D.wdcori = h->d_wdcori.as<double>();
   D.isstart = h->d_isstart.as<int>(); D.db = h->d_db.as<int>(); D.nsu = h->d_nsu.as<int>();
   D.nghl = h->nghl; D.RP = h->RP; D.dens = h->d_dens.as<double>(); D.act = act; D.n3 = n3;
   // few points: the pairs of a grid tile spread over psplit workgroups (psplit * na <= FAC_DPART)
@@ -2297,12 +2299,15 @@ int stage_hamiltonian_fac(pnfam_gpu_ctx *h, int na, const int *act, int n3) {
   { Prof pr(h, K_ZERO, na);
     hipLaunchKernelGGL(k_zero_slots, dim3(64, na), dim3(256), 0, h->stream, h->d_hsp.as<double>(), act, 8, 0, 8, h->nxy); }
   FacProjArgs P{};
-  P.PAh = h->PAh.as<__bf16>(); P.PAl = h->PAl.as<__bf16>(); P.pz = h->fac_pz;
-  P.psi[0] = h->d_wf.as<double>(); P.psi[1] = h->d_wfdr.as<double>(); P.psi[2] = h->d_wfdp.as<double>();
-  P.psi[3] = h->d_wfdz.as<double>(); P.psi[4] = h->d_wfd2.as<double>();
+  P.PAh = h->PAh.as<__bf16>(); P.PAl = h->PAl.as<__bf16>(); P.PS = h->PS32.as<float>(); P.pz = h->fac_pz;
   P.isstart = h->d_isstart.as<int>(); P.db = h->d_db.as<int>(); P.nsu = h->d_nsu.as<int>();
-  P.flds = h->d_flds.as<double>(); P.hsp = h->d_hsp.as<double>(); P.nxy = h->nxy; P.nghl = h->nghl; P.RP = h->RP;
+  P.FFm = h->FFm.as<float>(); P.FFp = h->FFp.as<float>();
+  P.hsp = h->d_hsp.as<double>(); P.nxy = h->nxy; P.nghl = h->nghl; P.RP = h->RP;
   P.act = act; P.n3 = n3;
+  { Prof pr(h, K_FAC_CVT, na);
+    const int nconv = h->RP * (FAC_FM + FAC_FP);
+    hipLaunchKernelGGL(k_fac_fconv, dim3((nconv + 255) / 256, na, 4), dim3(256), 0, h->stream, h->d_flds.as<double>(),
+                       h->nghl, h->RP, act, h->FFm.as<float>(), h->FFp.as<float>()); }
   { Prof pr(h, K_FAC_PROJ, na + 2 * n3);
     const Tile2 *tl = h->d_tiles2.as<Tile2>();
     // few points: one launch per kind over all row-tile classes (widest template)
