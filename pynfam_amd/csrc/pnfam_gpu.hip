@@ -1937,8 +1937,9 @@ int mix_workspace(pnfam_gpu_ctx *h, int cap) {
   size_t kmax = 0;
   for (int si = 0; si < 3; ++si) kmax = std::max(kmax, (size_t)h->mix[si].Kp);
   if (int e = h->d_Oproj.alloc(kmax * 4 * (size_t)cap * 4)) return e;
-  // split-K planes of the pairing-set density contractions
-  size_t psz = 0;
+  // split-K planes of the pairing-set density contractions and of the
+  // mean-field ones (at most four planes: dens_kchunk splits in three)
+  size_t psz = 4 * (size_t)h->mix[0].ntd * RP * 4 * (size_t)cap;
   for (int si = 1; si < 3; ++si) {
     const auto &S = h->mix[si];
     for (int c = 0; c < 4; ++c) {
@@ -1990,13 +1991,18 @@ int blas_split(pnfam_gpu_ctx *h, int64_t ns, int64_t nt, int64_t k, const __bf16
   return blas_nt(h, ns3, nt, k, Sl, lds, Th, ldt, C, ldc, true);
 }
 
-// Split-K form of blas_split for contractions with few output tiles (the pairing
-// densities: one grid-point row per table row): K chunks of MIX_KCHUNK go to
-// separate FP32 planes of P in one batched call each, the remainder to the next
-// plane, and k_sum_planes adds the planes into C.
+// Split-K form of blas_split for contractions with few output tiles: K chunks
+// of kchunk go to separate FP32 planes of P in one batched call each, the
+// remainder to the next plane, and k_sum_planes adds the first ns rows of every
+// plane column into C.  Used for the pairing densities (one grid-point row per
+// table row) and, with two or three chunks, for the mean-field densities when
+// the column count leaves rocBLAS too few output tiles for the device.
 
-__global__ void k_sum_planes(const float *__restrict__ P, int nplanes, size_t plane, float *__restrict__ C) {
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < plane; i += (size_t)gridDim.x * blockDim.x) {
+__global__ void k_sum_planes(const float *__restrict__ P, int nplanes, int64_t ns, int64_t nt, int64_t ldc,
+                             float *__restrict__ C) {
+  const size_t plane = (size_t)nt * ldc, n = (size_t)nt * ns;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+    const size_t i = (e / ns) * ldc + e % ns;
     float v = 0.0f;
     for (int j = 0; j < nplanes; ++j) v += P[(size_t)j * plane + i];
     C[i] = v;
@@ -2005,8 +2011,8 @@ __global__ void k_sum_planes(const float *__restrict__ P, int nplanes, size_t pl
 
 int blas_split_k(pnfam_gpu_ctx *h, int64_t ns, int64_t nt, int64_t k, const __bf16 *Sh, const __bf16 *Sl, int64_t lds,
                  const __bf16 *Th, const __bf16 *Tl, int64_t ldt, float *C, int64_t ldc, int64_t ns3, float *P,
-                 size_t pcap) {
-  const int64_t nfull = k / MIX_KCHUNK, rem = k - nfull * MIX_KCHUNK;
+                 size_t pcap, int64_t kchunk) {
+  const int64_t nfull = k / kchunk, rem = k - nfull * kchunk;
   const int64_t plane = nt * ldc;
   if ((size_t)(nfull + 1) * (size_t)plane > pcap) return fail(PNFAM_GPU_EINVAL, "split-K planes exceed their buffer");
   const __bf16 *Ss[3] = {Sh, Sh, Sl}, *Ts[3] = {Th, Tl, Th};
@@ -2014,16 +2020,31 @@ int blas_split_k(pnfam_gpu_ctx *h, int64_t ns, int64_t nt, int64_t k, const __bf
     const int64_t nsp = p == 0 ? ns : ns3;
     if (nsp == 0) break;
     if (nfull > 0)
-      if (int e = blas_nt_chunks(h, nsp, nt, MIX_KCHUNK, nfull, Ss[p], lds, Ts[p], ldt, P, ldc, plane, p > 0)) return e;
+      if (int e = blas_nt_chunks(h, nsp, nt, kchunk, nfull, Ss[p], lds, Ts[p], ldt, P, ldc, plane, p > 0)) return e;
     if (rem > 0)
-      if (int e = blas_nt(h, nsp, nt, rem, Ss[p] + nfull * MIX_KCHUNK, lds, Ts[p] + nfull * MIX_KCHUNK, ldt,
+      if (int e = blas_nt(h, nsp, nt, rem, Ss[p] + nfull * kchunk, lds, Ts[p] + nfull * kchunk, ldt,
                           P + nfull * plane, ldc, p > 0))
         return e;
   }
   const int np = (int)(nfull + (rem > 0 ? 1 : 0));
-  hipLaunchKernelGGL(k_sum_planes, dim3(1024), dim3(256), 0, h->stream, P, np, (size_t)plane, C);
+  const size_t n = (size_t)nt * ns;
+  hipLaunchKernelGGL(k_sum_planes, dim3((unsigned)std::min<size_t>(2048, (n + 255) / 256)), dim3(256), 0, h->stream, P,
+                     np, ns, nt, ldc, C);
   HIP_TRY(hipGetLastError());
   return 0;
+}
+
+// K chunk for a mean-field density contraction of ns columns: rocBLAS tiles
+// these shapes 256 columns wide, so below about 384 columns its grid leaves most
+// of the device idle; two or three K chunks in one batched call fill it (split
+// count measured on MI355X for 14400 table rows and K = 26112: 512 columns
+// 0.55 -> 0.37 ms with two, 256 columns 0.49 -> 0.21 ms with three).  The chunk
+// is rounded down to 64 so that the split count of chunks runs in one batched
+// call; the remainder (under 64 per chunk) takes one more small call.  0: no split.
+int64_t dens_kchunk(int64_t ns, int64_t k) {
+  const int nsplit = ns >= 384 ? 2 : (ns >= 96 ? 3 : 1);
+  if (nsplit == 1 || k < 4096) return 0;
+  return k / nsplit / 64 * 64;
 }
 
 void mix_gemm(pnfam_gpu_ctx *h, GemmArgs G, const pnfam_gpu_ctx::MixSet &S, int tmax) {
@@ -2070,7 +2091,12 @@ int stage_hamiltonian_mixed(pnfam_gpu_ctx *h, int na, const int *act, int n3) {
         if (si != 0) {   // pairing sets: one table row per grid point, too few output tiles
           if (int e = blas_split_k(h, nS, nT, kl, S.Bh.as<__bf16>() + k0, S.Bl.as<__bf16>() + k0, S.Kp,
                                    S.TDh.as<__bf16>() + k0, S.TDl.as<__bf16>() + k0, S.Kp, O, ncol, nS3,
-                                   h->d_Psplit.as<float>(), h->d_Psplit.bytes / 4))
+                                   h->d_Psplit.as<float>(), h->d_Psplit.bytes / 4, MIX_KCHUNK))
+            return e;
+        } else if (const int64_t kc = dens_kchunk(nS, kl)) {
+          if (int e = blas_split_k(h, nS, nT, kl, S.Bh.as<__bf16>() + k0, S.Bl.as<__bf16>() + k0, S.Kp,
+                                   S.TDh.as<__bf16>() + k0, S.TDl.as<__bf16>() + k0, S.Kp, O, ncol, nS3,
+                                   h->d_Psplit.as<float>(), h->d_Psplit.bytes / 4, kc))
             return e;
         } else if (int e = blas_split(h, nS, nT, kl, S.Bh.as<__bf16>() + k0, S.Bl.as<__bf16>() + k0, S.Kp,
                                       S.TDh.as<__bf16>() + k0, S.TDl.as<__bf16>() + k0, S.Kp, O, ncol, nS3))
