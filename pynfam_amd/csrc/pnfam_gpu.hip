@@ -1123,7 +1123,7 @@ __global__ void __launch_bounds__(64) k_bro_norm(BroArgs B) {
 // eight history vectors per pass over the new difference and vout
 constexpr int BRO_JB = 8;
 __global__ void __launch_bounds__(BRO_WG) k_bro_dots(BroArgs B) {
-  __shared__ double red[BRO_WG / 64];
+  __shared__ double red[2 * BRO_JB * (BRO_WG / 64)];
   const int w = B.act[blockIdx.y];
   const long N = B.N;
   const double normi = B.scal[(size_t)w * 4 + 0];
@@ -1151,12 +1151,24 @@ __global__ void __launch_bounds__(BRO_WG) k_bro_dots(BroArgs B) {
         }
       }
     }
+    // the 2 BRO_JB sums of the chunk together: per value the shuffle tree and
+    // wave order of wg_sum, one LDS exchange for all of them
 #pragma unroll
-    for (int j = 0; j < BRO_JB; ++j) {
-      if (j >= nj) break;
-      const double sg = wg_sum(g[j], red);
-      const double sv = wg_sum(v[j], red);
-      if (threadIdx.x == 0) { pp[2 * (j0 + j)] = sg; pp[2 * (j0 + j) + 1] = sv; }
+    for (int j = 0; j < BRO_JB; ++j)
+      for (int o = 32; o > 0; o >>= 1) {
+        g[j] += __shfl_down(g[j], o, 64);
+        v[j] += __shfl_down(v[j], o, 64);
+      }
+    const int lane = threadIdx.x % 64, wid = threadIdx.x / 64;
+    __syncthreads();
+    if (lane == 0)
+#pragma unroll
+      for (int j = 0; j < BRO_JB; ++j) { red[(2 * j) * (BRO_WG / 64) + wid] = g[j]; red[(2 * j + 1) * (BRO_WG / 64) + wid] = v[j]; }
+    __syncthreads();
+    if (threadIdx.x < 2 * nj) {
+      double t = 0.0;
+      for (int q = 0; q < BRO_WG / 64; ++q) t += red[threadIdx.x * (BRO_WG / 64) + q];
+      pp[2 * j0 + threadIdx.x] = t;
     }
   }
 }
@@ -1201,24 +1213,41 @@ __global__ void __launch_bounds__(256) k_bro_solve(BroArgs B) {
     a[i][j] = v;
   }
   __syncthreads();
+  // Gauss-Jordan with partial pivoting (the first largest |a[i][c]|), three
+  // barriers per column: wave 0 finds the pivot; the rows are swapped, row c
+  // scaled and the elimination factors taken in one pass; then the elimination.
   __shared__ int piv_s;
+  __shared__ double pval_s;
   for (int c = 0; c < k; ++c) {
-    if (threadIdx.x == 0) {
-      int piv = c;
-      for (int i = c + 1; i < k; ++i)
-        if (fabs(a[i][c]) > fabs(a[piv][c])) piv = i;
-      piv_s = piv;
+    if (threadIdx.x < 64) {
+      int bi = c;
+      double bv = -1.0;
+      for (int i = c + threadIdx.x; i < k; i += 64) {
+        const double x = fabs(a[i][c]);
+        if (x > bv) { bv = x; bi = i; }
+      }
+      for (int o = 32; o > 0; o >>= 1) {
+        const double ov = __shfl_xor(bv, o, 64);
+        const int oi = __shfl_xor(bi, o, 64);
+        if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
+      }
+      if (threadIdx.x == 0) { piv_s = bi; pval_s = a[bi][c]; }
     }
     __syncthreads();
+    // rows c and piv are written only by the column loop, which also records
+    // the factor of row piv (the old a[c][c]) from its own read
     const int piv = piv_s;
-    if (piv != c)
-      for (int j = threadIdx.x; j < 2 * k; j += blockDim.x) { double t = a[c][j]; a[c][j] = a[piv][j]; a[piv][j] = t; }
-    __syncthreads();
-    const double d = 1.0 / a[c][c];
-    __syncthreads();
-    for (int j = threadIdx.x; j < 2 * k; j += blockDim.x) a[c][j] *= d;
-    __syncthreads();
-    for (int i = threadIdx.x; i < k; i += blockDim.x) fac[i] = a[i][c];
+    const double d = 1.0 / pval_s;
+    for (int i = threadIdx.x; i < k; i += blockDim.x)
+      if (i != piv && i != c) fac[i] = a[i][c];
+    for (int j = threadIdx.x; j < 2 * k; j += blockDim.x) {
+      const double t = a[c][j], pv = a[piv][j];
+      if (piv != c) {
+        if (j == c) fac[piv] = t;
+        a[piv][j] = t;
+      }
+      a[c][j] = pv * d;
+    }
     __syncthreads();
     for (int e = threadIdx.x; e < k * 2 * k; e += blockDim.x) {
       const int i = e / (2 * k), j = e % (2 * k);
@@ -1252,6 +1281,7 @@ __global__ void __launch_bounds__(BRO_WG) k_bro_update(BroArgs B, int mode) {
       c = B.alpha * r;
     } else {
       c = B.alpha * r;
+#pragma unroll 8
       for (int i = 0; i < B.iter_used; ++i) {
         const size_t o = ((size_t)w * B.M + i) * N + e;
         c -= gm[i] * (B.dv[o] + B.alpha * B.df[o]);
