@@ -1407,6 +1407,11 @@ struct pnfam_gpu_ctx {
   int tpa_first[2][TP_MAXCT + 1] = {}, tpa_count[2][TP_MAXCT + 1] = {}, tpa_strips[2][TP_MAXCT + 1] = {};
   long tp_L[2] = {0, 0}, tp_Lcap = 0;                        // per active point, per stage
   int unit_precision = PNFAM_GPU_FP64;   // grid stage of the per-unit entry points
+  // pinned host staging of the per-iteration transfers: active lists [3][cap]
+  // (all, FP64, low precision) and results [4][cap] (si, vmax, re S, im S)
+  int *pin_act = nullptr;
+  double *pin_res = nullptr;
+  int pin_cap = 0;
   int bro_nwg = 0;
   double flops_q2s = 0, flops_s2q = 0;
   // per-kernel event timing (pnfam_gpu_profile)
@@ -1419,6 +1424,8 @@ struct pnfam_gpu_ctx {
   double flops_unit[32] = {0};   // algorithmic FLOP per frequency, per kernel id
   ~pnfam_gpu_ctx() {
     if (blas) (void)rocblas_destroy_handle(blas);
+    if (pin_act) (void)hipHostFree(pin_act);
+    if (pin_res) (void)hipHostFree(pin_res);
     for (auto &p : pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
     for (auto e : event_pool) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
@@ -1723,6 +1730,15 @@ int ensure_workspace(pnfam_gpu_ctx *h, int npts, int M) {
   if ((e = h->d_coef.alloc(n * Mh * 8))) return e;
   if ((e = h->d_wre.alloc(n * 8))) return e;
   if ((e = h->d_wim.alloc(n * 8))) return e;
+  if ((int)n > h->pin_cap) {
+    if (h->pin_act) { (void)hipHostFree(h->pin_act); h->pin_act = nullptr; }
+    if (h->pin_res) { (void)hipHostFree(h->pin_res); h->pin_res = nullptr; }
+    h->pin_cap = 0;
+    if (hipHostMalloc((void **)&h->pin_act, 3 * n * 4) != hipSuccess ||
+        hipHostMalloc((void **)&h->pin_res, 4 * n * 8) != hipSuccess)
+      return fail(PNFAM_GPU_ENOMEM, "hipHostMalloc of the pinned staging buffers failed");
+    h->pin_cap = (int)n;
+  }
   if ((e = h->d_act.alloc(n * 4))) return e;
   if ((e = h->d_act64.alloc(n * 4))) return e;
   if ((e = h->d_actlo.alloc(n * 4))) return e;
@@ -2968,13 +2984,15 @@ int pnfam_gpu_solve_batch(pnfam_gpu_ctx *h, int32_t npts, const double *re_w, co
     if (act.empty()) break;
     const int na = (int)act.size();
     int *dact = h->d_act.as<int>();
-    HIP_TRY(hipMemcpyAsync(dact, act.data(), na * 4, hipMemcpyHostToDevice, st));
+    std::memcpy(h->pin_act, act.data(), na * 4);
+    HIP_TRY(hipMemcpyAsync(dact, h->pin_act, na * 4, hipMemcpyHostToDevice, st));
     // vin = 0 at iteration 0, so H(vin) = 0 exactly there and the chain is not run
     const bool use_h = !skip_res && iter > 0;
     if (use_h && !act64.empty()) {
       const int n64 = (int)act64.size();
       int *d64 = h->d_act64.as<int>();
-      HIP_TRY(hipMemcpyAsync(d64, act64.data(), n64 * 4, hipMemcpyHostToDevice, st));
+      std::memcpy(h->pin_act + npts, act64.data(), n64 * 4);
+      HIP_TRY(hipMemcpyAsync(d64, h->pin_act + npts, n64 * 4, hipMemcpyHostToDevice, st));
       if (int e = stage_qp2sp(h, h->d_vin.as<double>(), n64, d64)) return e;
       if (int e = stage_hamiltonian(h, n64, d64)) return e;
       if (int e = stage_sp2qp(h, n64, d64)) return e;
@@ -2996,7 +3014,8 @@ int pnfam_gpu_solve_batch(pnfam_gpu_ctx *h, int32_t npts, const double *re_w, co
       n3_last = n3;
       const int nlo = (int)actlo.size();
       int *dlo = h->d_actlo.as<int>();
-      HIP_TRY(hipMemcpyAsync(dlo, actlo.data(), nlo * 4, hipMemcpyHostToDevice, st));
+      std::memcpy(h->pin_act + 2 * npts, actlo.data(), nlo * 4);
+      HIP_TRY(hipMemcpyAsync(dlo, h->pin_act + 2 * npts, nlo * 4, hipMemcpyHostToDevice, st));
       const bool tpl = h->tp_bf16 && !h->basic_kernels;
       if (int e = stage_qp2sp(h, h->d_dvin.as<double>(), nlo, dlo, tpl, n3)) return e;
       if (int e = (h->basic_kernels ? stage_hamiltonian(h, nlo, dlo) : stage_hamiltonian_lowp(h, nlo, dlo, n3)))
@@ -3047,10 +3066,14 @@ int pnfam_gpu_solve_batch(pnfam_gpu_ctx *h, int32_t npts, const double *re_w, co
     { Prof pr(h, K_CONTRACT_FIN, na);
       hipLaunchKernelGGL(k_contract_fin, dim3(na), dim3(64), 0, st, h->d_cpart.as<double>(), cnwg, h->d_S.as<double>(), dact); }
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipMemcpyAsync(si.data(), h->d_si.p, npts * 8, hipMemcpyDeviceToHost, st));
-    if (mixed) HIP_TRY(hipMemcpyAsync(vmax.data(), h->d_vmax.p, npts * 8, hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipMemcpyAsync(S.data(), h->d_S.p, 2 * npts * 8, hipMemcpyDeviceToHost, st));
+    double *pr = h->pin_res;
+    HIP_TRY(hipMemcpyAsync(pr, h->d_si.p, npts * 8, hipMemcpyDeviceToHost, st));
+    if (mixed) HIP_TRY(hipMemcpyAsync(pr + npts, h->d_vmax.p, npts * 8, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(pr + 2 * npts, h->d_S.p, 2 * npts * 8, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
+    std::memcpy(si.data(), pr, npts * 8);
+    if (mixed) std::memcpy(vmax.data(), pr + npts, npts * 8);
+    std::memcpy(S.data(), pr + 2 * npts, 2 * npts * 8);
     if (h->prof) prof_collect(h);
     for (int w : act) { re_s[w] = S[2 * w]; im_s[w] = S[2 * w + 1]; }
     if (trace) {
