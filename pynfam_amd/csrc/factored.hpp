@@ -678,37 +678,61 @@ __global__ void k_tp_copy(const double *__restrict__ m, const int4 *__restrict__
 // staged through LDS so that the FP64 reads run along B's storage order
 // (j for a transposed block, k otherwise) and the writes along k.
 constexpr int TP_TJ = 16, TP_TK = 32;
+// One 16 x 32 tile (TP_TJ columns j of op(B) by TP_TK values of k) per wave and
+// job; the waves of a workgroup take jobs blockIdx.x * 4 + wave + 4 gridDim.x q.
+// Loads follow B's storage order, the tile is transposed through the wave's own
+// LDS slice, and each lane writes 8 consecutive k of one column as one 16-byte
+// hi and one 16-byte lo store (term offsets and the slot stride are multiples
+// of 512 elements).
 __global__ void __launch_bounds__(256) k_tp_bconv(const Term *__restrict__ terms, const int *__restrict__ toff,
-                                                  const int2 *__restrict__ jobs, const double *__restrict__ bbase,
-                                                  long bstride, long nxy, const int *__restrict__ act, long L,
-                                                  __bf16 *__restrict__ oh, __bf16 *__restrict__ ol) {
-  __shared__ float tile[TP_TJ][TP_TK + 1];
-  const int2 jb = jobs[blockIdx.x];          // (term, tile)
+                                                  const int2 *__restrict__ jobs, int njobs,
+                                                  const double *__restrict__ bbase, long bstride, long nxy,
+                                                  const int *__restrict__ act, long L, __bf16 *__restrict__ oh,
+                                                  __bf16 *__restrict__ ol) {
+  __shared__ float tile[4][TP_TJ][TP_TK + 1];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int slot = blockIdx.y;
   const int w = act[slot];
-  const Term tm = terms[jb.x];
-  const int nda = tm.nda, ndc = tm.ndc;
-  const int Pa = (nda + 31) & ~31;
-  const int ntk = Pa / TP_TK;
-  const int j0 = (jb.y / ntk) * TP_TJ, k0 = (jb.y % ntk) * TP_TK;
-  const double *Bb = bbase + (size_t)w * bstride + (size_t)tm.b_slot * nxy + tm.b_off;
-  for (int idx = threadIdx.x; idx < TP_TJ * TP_TK; idx += blockDim.x) {
-    int jj, kk;
-    if (tm.tb) { jj = idx % TP_TJ; kk = idx / TP_TJ; }
-    else { kk = idx % TP_TK; jj = idx / TP_TK; }
-    const int j = j0 + jj, k = k0 + kk;
-    float v = 0.0f;
-    if (j < ndc && k < nda) v = (float)(tm.tb ? Bb[j + (size_t)k * tm.ldb] : Bb[k + (size_t)j * tm.ldb]);
-    tile[jj][kk] = v;
-  }
-  __syncthreads();
-  const size_t o = (size_t)slot * L + toff[jb.x];
-  for (int idx = threadIdx.x; idx < TP_TJ * TP_TK; idx += blockDim.x) {
-    const int jj = idx / TP_TK, kk = idx % TP_TK;
-    __bf16 h, l;
-    split_bf16(tile[jj][kk], h, l);
-    const size_t e = o + (size_t)(j0 + jj) * Pa + k0 + kk;
-    oh[e] = h;
-    ol[e] = l;
+  float(*tl)[TP_TK + 1] = tile[wave];
+  for (int job = blockIdx.x * 4 + wave; job < njobs; job += gridDim.x * 4) {
+    const int2 jb = jobs[job];          // (term, tile)
+    const Term tm = terms[jb.x];
+    const int nda = tm.nda, ndc = tm.ndc;
+    const int Pa = (nda + 31) & ~31;
+    const int ntk = Pa / TP_TK;
+    const int j0 = (jb.y / ntk) * TP_TJ, k0 = (jb.y % ntk) * TP_TK;
+    const double *Bb = bbase + (size_t)w * bstride + (size_t)tm.b_slot * nxy + tm.b_off;
+#pragma unroll
+    for (int q = 0; q < TP_TJ * TP_TK / 64; ++q) {
+      const int idx = lane + 64 * q;
+      int jj, kk;
+      if (tm.tb) { jj = idx % TP_TJ; kk = idx / TP_TJ; }
+      else { kk = idx % TP_TK; jj = idx / TP_TK; }
+      const int j = j0 + jj, k = k0 + kk;
+      float v = 0.0f;
+      if (j < ndc && k < nda) v = (float)(tm.tb ? Bb[j + (size_t)k * tm.ldb] : Bb[k + (size_t)j * tm.ldb]);
+      tl[jj][kk] = v;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    {
+      const int jj = lane >> 2, kq = (lane & 3) * 8;
+      v8bf hv, lv;
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        __bf16 h, l;
+        split_bf16(tl[jj][kq + i], h, l);
+        hv[i] = h;
+        lv[i] = l;
+      }
+      const size_t e = (size_t)slot * L + toff[jb.x] + (size_t)(j0 + jj) * Pa + k0 + kq;
+      *(v8bf *)(oh + e) = hv;
+      *(v8bf *)(ol + e) = lv;
+    }
+    // the slice is rewritten by the next job: every lane's reads come first
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
 }

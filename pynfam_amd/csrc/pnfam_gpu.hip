@@ -1664,10 +1664,13 @@ int launch_triprod_bf16(pnfam_gpu_ctx *h, int stage, const double *bbase, long b
   if (na > h->cap) return fail(PNFAM_GPU_EINVAL, "more active points than the batch capacity");
   const Term *terms = h->d_tpa_terms[stage].as<Term>();
   const Group *groups = h->d_tpa_groups[stage].as<Group>();
-  if (h->tpa_njobs[stage])
-    hipLaunchKernelGGL(k_tp_bconv, dim3(h->tpa_njobs[stage], na), dim3(256), 0, h->stream, terms,
-                       h->d_tpa_toff[stage].as<int>(), h->d_tpa_jobs[stage].as<int2>(), bbase, bstride, h->nxy,
+  if (const int nj = h->tpa_njobs[stage]) {
+    // four waves per workgroup; with many points each wave takes four jobs
+    const int per = na >= 16 ? 16 : 4;
+    hipLaunchKernelGGL(k_tp_bconv, dim3((nj + per - 1) / per, na), dim3(256), 0, h->stream, terms,
+                       h->d_tpa_toff[stage].as<int>(), h->d_tpa_jobs[stage].as<int2>(), nj, bbase, bstride, h->nxy,
                        act, L, h->tp_Bh.as<__bf16>(), h->tp_Bl.as<__bf16>());
+  }
   TpMats M{h->tp_rmh.as<__bf16>(), h->tp_rml.as<__bf16>(), h->tp_cmh.as<__bf16>(), h->tp_cml.as<__bf16>(),
            h->tp_mstride};
   TpB BB{h->tp_Bh.as<__bf16>(), h->tp_Bl.as<__bf16>(), L, h->d_tpa_toff[stage].as<int>()};
