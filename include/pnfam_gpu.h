@@ -72,6 +72,8 @@ typedef struct {
  *     and the point iterates on until it meets the tolerance again.
  *   MIXED_NOREFINE: as MIXED without the final FP64 evaluation; S then carries
  *     the low-precision error of the first increments (about 1e-6 relative).
+ *   Both mixed modes keep the Broyden history in FP32 (environment
+ *   PNFAM_GPU_MIX_HIST=fp64 keeps it in FP64); the residual stays FP64.
  *   In both mixed modes a point whose increment, judged by its last residual,
  *   is below 3e-3 of its largest amplitude (environment PNFAM_GPU_MIX_TAU)
  *   goes through single BF16 products instead of the three split products:

@@ -1025,7 +1025,8 @@ constexpr int BRO_MAXM = 64;
 
 struct BroArgs {
   double *vin, *vout;     // [w][N]
-  double *df, *dv;        // [w][M][N]
+  void *df, *dv;          // history [w][M][N]: double, or float on the mixed path
+  double *pdf, *pdv;      // float history: the pending raw vout and vin [w][N] (FP64)
   double *part;           // [w][nwg][2*M+2]
   double *gram;           // [w][M][M]
   double *coef;           // [w][M] gamma
@@ -1064,15 +1065,21 @@ __device__ inline double wg_max(double v, double *red) {
 
 // vout <- vout - vin; partial max|vout|; for iter >= 2 also finalise the history
 // difference at slot ipos and its partial squared norm.
+// H: element type of the history (double; float on the mixed path, whose
+// pending raw vout / vin stay FP64 in pdf / pdv so the differences are exact).
+template <class H>
 __global__ void __launch_bounds__(BRO_WG) k_bro_resid(BroArgs B) {
+  constexpr bool f64 = sizeof(H) == 8;
   __shared__ double red[BRO_WG / 64];
   const int w = B.act[blockIdx.y];
   const long N = B.N;
   double *vo = B.vout + (size_t)w * N;
   const double *vi = B.vin + (size_t)w * N;
   const bool hist = B.iter >= 2 && B.M > 0;
-  double *dfp = B.df + ((size_t)w * B.M + (hist ? B.ipos : 0)) * N;
-  double *dvp = B.dv + ((size_t)w * B.M + (hist ? B.ipos : 0)) * N;
+  H *dfp = (H *)B.df + ((size_t)w * B.M + (hist ? B.ipos : 0)) * N;
+  H *dvp = (H *)B.dv + ((size_t)w * B.M + (hist ? B.ipos : 0)) * N;
+  const double *qf = f64 ? nullptr : B.pdf + (size_t)w * N;
+  const double *qv = f64 ? nullptr : B.pdv + (size_t)w * N;
   double mx = 0.0, nr = 0.0, vm = 0.0;
   for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < N; e += (long)gridDim.x * blockDim.x) {
     const double d = vo[e] - vi[e];
@@ -1080,9 +1087,9 @@ __global__ void __launch_bounds__(BRO_WG) k_bro_resid(BroArgs B) {
     mx = fmax(mx, fabs(d));
     vm = fmax(vm, fabs(vi[e]));
     if (hist) {
-      const double f = d - dfp[e];
-      dfp[e] = f;
-      dvp[e] = vi[e] - dvp[e];
+      const double f = d - (f64 ? (double)dfp[e] : qf[e]);
+      dfp[e] = (H)f;
+      dvp[e] = (H)(vi[e] - (f64 ? (double)dvp[e] : qv[e]));
       nr += f * f;
     }
   }
@@ -1122,24 +1129,25 @@ __global__ void __launch_bounds__(64) k_bro_norm(BroArgs B) {
 // scale the history slot, then partial dots df_j . df_ipos and df_j . vout,
 // eight history vectors per pass over the new difference and vout
 constexpr int BRO_JB = 8;
+template <class H>
 __global__ void __launch_bounds__(BRO_WG) k_bro_dots(BroArgs B) {
   __shared__ double red[2 * BRO_JB * (BRO_WG / 64)];
   const int w = B.act[blockIdx.y];
   const long N = B.N;
   const double normi = B.scal[(size_t)w * 4 + 0];
-  double *dfp = B.df + ((size_t)w * B.M + B.ipos) * N;
-  double *dvp = B.dv + ((size_t)w * B.M + B.ipos) * N;
+  H *dfp = (H *)B.df + ((size_t)w * B.M + B.ipos) * N;
+  H *dvp = (H *)B.dv + ((size_t)w * B.M + B.ipos) * N;
   const double *vo = B.vout + (size_t)w * N;
   const long e0 = (long)blockIdx.x * blockDim.x + threadIdx.x, es = (long)gridDim.x * blockDim.x;
   if (B.iter >= 2)
-    for (long e = e0; e < N; e += es) { dfp[e] *= normi; dvp[e] *= normi; }
+    for (long e = e0; e < N; e += es) { dfp[e] = (H)(dfp[e] * normi); dvp[e] = (H)(dvp[e] * normi); }
   double *pp = B.part + ((size_t)w * B.nwg + blockIdx.x) * (2 * BRO_MAXM + 2) + 2;
   for (int j0 = 0; j0 < B.iter_used; j0 += BRO_JB) {
     const int nj = min(BRO_JB, B.iter_used - j0);
     double g[BRO_JB], v[BRO_JB];
 #pragma unroll
     for (int j = 0; j < BRO_JB; ++j) { g[j] = 0.0; v[j] = 0.0; }
-    const double *df0 = B.df + ((size_t)w * B.M + j0) * N;
+    const H *df0 = (const H *)B.df + ((size_t)w * B.M + j0) * N;
     for (long e = e0; e < N; e += es) {
       const double p = dfp[e], o = vo[e];
 #pragma unroll
@@ -1265,7 +1273,10 @@ __global__ void __launch_bounds__(256) k_bro_solve(BroArgs B) {
 // curv = alpha vout - sum_i gamma_i (dv_i + alpha df_i); df[inext] = vout;
 // dv[inext] = vin; vin += curv.  With iter_used = 0 this is also the first
 // Broyden step; mode 1/2 give linear mixing / no mixing.
+template <class H>
 __global__ void __launch_bounds__(BRO_WG) k_bro_update(BroArgs B, int mode) {
+  constexpr bool f64 = sizeof(H) == 8;
+  const H *hdf = (const H *)B.df, *hdv = (const H *)B.dv;
   const int w = B.act[blockIdx.y];
   const long N = B.N;
   double *vi = B.vin + (size_t)w * N;
@@ -1284,11 +1295,16 @@ __global__ void __launch_bounds__(BRO_WG) k_bro_update(BroArgs B, int mode) {
 #pragma unroll 8
       for (int i = 0; i < B.iter_used; ++i) {
         const size_t o = ((size_t)w * B.M + i) * N + e;
-        c -= gm[i] * (B.dv[o] + B.alpha * B.df[o]);
+        c -= gm[i] * ((double)hdv[o] + B.alpha * (double)hdf[o]);
       }
-      const size_t on = ((size_t)w * B.M + B.inext) * N + e;
-      B.df[on] = r;
-      B.dv[on] = vi[e];
+      if (f64) {
+        const size_t on = ((size_t)w * B.M + B.inext) * N + e;
+        ((double *)B.df)[on] = r;
+        ((double *)B.dv)[on] = vi[e];
+      } else {
+        B.pdf[(size_t)w * N + e] = r;
+        B.pdv[(size_t)w * N + e] = vi[e];
+      }
     }
     vi[e] += c;
     if (dv_out) dv_out[e] = c;
@@ -1359,7 +1375,7 @@ struct pnfam_gpu_ctx {
   int cap = 0;
   int hist_cap = -1;
   DevPtr d_vin, d_vout, d_rsp, d_hsp, d_hqp, d_dens, d_flds, d_part, d_cpart, d_S, d_si, d_vmax, d_scal;
-  DevPtr d_df, d_dv, d_gram, d_coef, d_wre, d_wim, d_act;
+  DevPtr d_df, d_dv, d_pdf, d_pdv, d_gram, d_coef, d_wre, d_wim, d_act;
   // mixed path: the next increment of vin, accumulated FP64 H20/H02
   DevPtr d_act64, d_actlo, d_dvin, d_hacc;
   // split-BF16 grid stage: one element layout per block structure (the mean-field
@@ -1729,6 +1745,8 @@ int ensure_workspace(pnfam_gpu_ctx *h, int npts, int M) {
   if ((e = h->d_scal.alloc(n * 4 * 8))) return e;
   if ((e = h->d_df.alloc(n * Mh * 4 * nxy * 8))) return e;
   if ((e = h->d_dv.alloc(n * Mh * 4 * nxy * 8))) return e;
+  if ((e = h->d_pdf.alloc(n * 4 * nxy * 8))) return e;
+  if ((e = h->d_pdv.alloc(n * 4 * nxy * 8))) return e;
   if ((e = h->d_gram.alloc(n * Mh * Mh * 8))) return e;
   if ((e = h->d_coef.alloc(n * Mh * 8))) return e;
   if ((e = h->d_wre.alloc(n * 8))) return e;
@@ -2958,6 +2976,9 @@ int pnfam_gpu_solve_batch(pnfam_gpu_ctx *h, int32_t npts, const double *re_w, co
   std::vector<int> done(npts, 0), refined(npts, 0), act, act64, actlo;
   double tau = 3e-3;   // one-product threshold of the mixed path
   if (const char *ev = std::getenv("PNFAM_GPU_MIX_TAU")) tau = std::atof(ev);
+  // the mixed path keeps the Broyden history in FP32 (PNFAM_GPU_MIX_HIST=fp64: FP64)
+  bool hist32 = mixed;
+  if (const char *ev = std::getenv("PNFAM_GPU_MIX_HIST")) hist32 = mixed && std::strcmp(ev, "fp64") != 0;
   for (int w = 0; w < npts; ++w) { iters[w] = -1; re_s[w] = 0.0; im_s[w] = 0.0; }
   const int nwg = h->bro_nwg;
   const long nq = 4 * nxy;   // doubles of H20/H02 (re, im) per frequency
@@ -3039,8 +3060,9 @@ int pnfam_gpu_solve_batch(pnfam_gpu_ctx *h, int32_t npts, const double *re_w, co
       hipLaunchKernelGGL(k_green, dim3((unsigned)std::min<long>(1024, (2 * nxy + 255) / 256), na), dim3(256), 0, st, G); }
     // Broyden
     BroArgs B{};
-    B.vin = h->d_vin.as<double>(); B.vout = h->d_vout.as<double>(); B.df = h->d_df.as<double>();
-    B.dv = h->d_dv.as<double>(); B.part = h->d_part.as<double>(); B.gram = h->d_gram.as<double>();
+    B.vin = h->d_vin.as<double>(); B.vout = h->d_vout.as<double>(); B.df = h->d_df.p; B.dv = h->d_dv.p;
+    B.pdf = h->d_pdf.as<double>(); B.pdv = h->d_pdv.as<double>();
+    B.part = h->d_part.as<double>(); B.gram = h->d_gram.as<double>();
     B.coef = h->d_coef.as<double>(); B.si = h->d_si.as<double>(); B.scal = h->d_scal.as<double>();
     B.vmax = h->d_vmax.as<double>();
     B.dvin = mixed ? h->d_dvin.as<double>() : nullptr;
@@ -3055,13 +3077,19 @@ int pnfam_gpu_solve_batch(pnfam_gpu_ctx *h, int32_t npts, const double *re_w, co
       if (iter < 2) B.ipos = 0;
     }
     B.iter = mode == 0 ? iter : 0;
-    { Prof pr(h, K_BRO_RESID, na); hipLaunchKernelGGL(k_bro_resid, dim3(nwg, na), dim3(BRO_WG), 0, st, B); }
+    { Prof pr(h, K_BRO_RESID, na);
+      if (hist32) hipLaunchKernelGGL(k_bro_resid<float>, dim3(nwg, na), dim3(BRO_WG), 0, st, B);
+      else hipLaunchKernelGGL(k_bro_resid<double>, dim3(nwg, na), dim3(BRO_WG), 0, st, B); }
     { Prof pr(h, K_BRO_NORM, na); hipLaunchKernelGGL(k_bro_norm, dim3(na), dim3(64), 0, st, B); }
     if (mode == 0) {
-      { Prof pr(h, K_BRO_DOTS, na); hipLaunchKernelGGL(k_bro_dots, dim3(nwg, na), dim3(BRO_WG), 0, st, B); }
+      { Prof pr(h, K_BRO_DOTS, na);
+        if (hist32) hipLaunchKernelGGL(k_bro_dots<float>, dim3(nwg, na), dim3(BRO_WG), 0, st, B);
+        else hipLaunchKernelGGL(k_bro_dots<double>, dim3(nwg, na), dim3(BRO_WG), 0, st, B); }
       { Prof pr(h, K_BRO_SOLVE, na); hipLaunchKernelGGL(k_bro_solve, dim3(na), dim3(256), 0, st, B); }
     }
-    { Prof pr(h, K_BRO_UPDATE, na); hipLaunchKernelGGL(k_bro_update, dim3(nwg, na), dim3(BRO_WG), 0, st, B, mode); }
+    { Prof pr(h, K_BRO_UPDATE, na);
+      if (hist32) hipLaunchKernelGGL(k_bro_update<float>, dim3(nwg, na), dim3(BRO_WG), 0, st, B, mode);
+      else hipLaunchKernelGGL(k_bro_update<double>, dim3(nwg, na), dim3(BRO_WG), 0, st, B, mode); }
     const int cnwg = 64;
     { Prof pr(h, K_CONTRACT, na);
       hipLaunchKernelGGL(k_contract, dim3(cnwg, na), dim3(256), 0, st, h->d_vin.as<double>(), h->d_f20.as<double>(),
