@@ -307,8 +307,13 @@ __global__ void k_mix_fields(CvtFldArgs A) {
   }
   for (int sec = 0; sec < 2; ++sec) {
     const double *f = A.flds + (((size_t)w * 2 + sec) * 2 * NFLD) * A.nghl;
+    // unrolled: the combination and class are compile-time, so proj_field
+    // reduces to its loads (a runtime switch kept the fields in scratch)
+#pragma unroll
     for (int c = 0; c < NPROJ_C; ++c) {
+#pragma unroll
       for (int cls = 0; cls < 4; ++cls) {
+#pragma unroll
         for (int ri = 0; ri < 2; ++ri) {
           const float v = live ? (float)(2.0 * proj_field(cls, c, f, A.nghl, r, ri)) : 0.0f;
           __bf16 h, l;
