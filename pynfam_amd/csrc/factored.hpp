@@ -141,6 +141,16 @@ __device__ inline v4f mfma_split(v8bf ah, v8bf al, v8bf bh, v8bf bl, v4f c, bool
 
 constexpr int FAC_NW = 4;     // waves per workgroup of the factored kernels
 
+// One-dimensional grids whose units come in chunks of C consecutive ids that
+// share operands: workgroup L runs on XCD L % 8, and this mapping gives XCD x
+// the units [x C, x C + C), then [8 C + x C, ...), so a chunk's workgroups run
+// together on one XCD and read the shared data through one L2.  Grid size:
+// ceil(units / (8 C)) * 8 C; ids at or beyond the unit count exit.
+__device__ inline int xcd_chunk_unit(int L, int C) {
+  const int xcd = L & 7, loc = L >> 3;
+  return (loc / C) * (8 * C) + xcd * C + loc % C;
+}
+
 // One workgroup of FAC_NW waves per (16 grid points, frequency, sector, part);
 // the block pairs are dealt round-robin to the FAC_NW x psplit waves of the
 // psplit parts, so a frequency spreads over that many waves.  The waves of a
@@ -325,6 +335,7 @@ struct FacProjArgs {
   int nghl, RP;
   const int *act;
   int n3;                         // active slots below n3 take three BF16 products
+  int na, ntiles, chunk;          // grid: ntiles x na units in XCD chunks of `chunk`
 };
 
 // The z' of each z with a non-zero mean field (pnfam_hamiltonian_blas.f90:
@@ -380,16 +391,21 @@ __global__ void __launch_bounds__(64 * FAC_NW, 2) k_fac_project(FacProjArgs P, c
   constexpr int RED = (FAC_NW - 1) * NRT * 8 * 64;    // reduction floats
   constexpr int SM = FAC_NW * STG > RED ? FAC_NW * STG : RED;
   __shared__ __attribute__((aligned(16))) float smem[SM];
-  const Tile2 t = tiles[blockIdx.x];
+  // unit = (tile, slot), slots fastest; chunks of consecutive tiles share the
+  // psi rows of their row group and the columns of their column tile
+  const int unit = xcd_chunk_unit(blockIdx.x, P.chunk);
+  if (unit >= P.ntiles * P.na) return;
+  const int slot = unit % P.na;
+  const Tile2 t = tiles[unit / P.na];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int fi = lane & 15, fg = lane >> 4;
   float *sMF = smem + wave * STG;                      // [r][s_b][combo][ri]
   const int ia = P.isstart[t.ix], ib0 = P.isstart[t.iy];
   const int dx = P.db[t.ix], dy = P.db[t.iy];
   const int nrows = t.a_hi - t.a_lo;
-  const bool three = (int)blockIdx.y < P.n3;
+  const bool three = slot < P.n3;
   // this tile's staged fields: STG consecutive floats per 32-point step
-  const float *ff = (KIND == 0 ? P.FFm : P.FFp) + (((size_t)blockIdx.y * 2 + t.sec) * 2 + t.sa) * P.RP * (STG / FAC_RT);
+  const float *ff = (KIND == 0 ? P.FFm : P.FFp) + (((size_t)slot * 2 + t.sec) * 2 + t.sa) * P.RP * (STG / FAC_RT);
   const int bj = min(t.b0 + fi, dy - 1);           // this lane's column (clamped; outputs masked)
   const int sb = bj < P.nsu[t.iy] ? 0 : 1;
   const float *pcol = P.PS + (size_t)(ib0 + bj) * P.RP;
@@ -483,7 +499,7 @@ __global__ void __launch_bounds__(64 * FAC_NW, 2) k_fac_project(FacProjArgs P, c
 #pragma unroll
         for (int v = 0; v < 4; ++v) C[q][ri][v] += red[((q * 2 + ri) * 4 + v) * 64 + lane];
   }
-  const int w = P.act[blockIdx.y];
+  const int w = P.act[slot];
   double *ore = P.hsp + ((size_t)w * 8 + 2 * t.out_slot) * P.nxy + t.off;
   double *oim = ore + P.nxy;
   const int bcol = t.b0 + fi;

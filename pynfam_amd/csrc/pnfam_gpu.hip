@@ -2418,7 +2418,10 @@ int stage_hamiltonian_fac(pnfam_gpu_ctx *h, int na, const int *act, int n3) {
       if (merge && k != mk[kind]) continue;
       const int n = merge ? mcount[kind] : h->pj_count[k];
       if (!n) continue;
-      const dim3 grid(n, na);
+      // XCD chunks of eight consecutive tiles for every active slot
+      P.na = na; P.ntiles = n; P.chunk = std::min(1024, 8 * na);
+      const long units = (long)n * na, per = 8L * P.chunk;
+      const dim3 grid((unsigned)((units + per - 1) / per * per));
       const Tile2 *tk = tl + (merge ? mfirst[kind] : h->pj_first[k]);
 #define FP_LAUNCH(NRT, KIND) hipLaunchKernelGGL((k_fac_project<NRT, KIND>), grid, dim3(64 * FAC_NW), 0, h->stream, P, tk)
       switch (k) {
