@@ -552,7 +552,10 @@ __device__ inline void wave_sync() {
 // The grid is one-dimensional: workgroup L runs on XCD L % 8, and the strips of
 // one (group, point) unit get consecutive ids on one XCD, so they read the
 // unit's B operand through the same L2.
-template <int NCT>
+// SW: the waves of a workgroup take four consecutive strips of one unit
+// instead, each summing every term itself; the four waves then read the same B
+// and C fragments at about the same time, mostly from the CU's L1.
+template <int NCT, bool SW>
 __global__ void __launch_bounds__(64 * TP_NW) k_triprod_bf16(const Group *__restrict__ groups, int ngroups,
                                                              int nstrip, const Term *__restrict__ terms, TpMats M,
                                                              TpB BB, double *__restrict__ obase, long ostride,
@@ -560,17 +563,18 @@ __global__ void __launch_bounds__(64 * TP_NW) k_triprod_bf16(const Group *__rest
   constexpr int TW = (16 * NCT + 31) / 32 * 32;   // strip width padded to whole K steps
   constexpr int PT = TW + 4;
   __shared__ __attribute__((aligned(16))) float smem[TP_NW * 16 * PT];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int xcd = blockIdx.x & 7, loc = blockIdx.x >> 3;
-  const int strip = loc % nstrip;
-  const int unit = (loc / nstrip) * 8 + xcd;
+  const int nq = SW ? (nstrip + TP_NW - 1) / TP_NW : nstrip;   // workgroups per unit
+  const int strip = SW ? (loc % nq) * TP_NW + wave : loc % nq;
+  const int unit = (loc / nq) * 8 + xcd;
   if (unit >= ngroups * na) return;                // uniform per workgroup
   const int gi = unit % ngroups, slot = unit / ngroups;
   const Group g = groups[gi];
   const int w = act[slot];
   const bool three = slot < n3;
   const int i0 = strip * 16;
-  if (i0 >= g.nrow) return;                       // uniform per workgroup
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (i0 >= g.nrow) return;   // uniform per workgroup; per wave with SW, which has no block barriers
   const int fi = lane & 15, fg = lane >> 4;
   float *sT = smem + wave * 16 * PT;
   for (int e = lane; e < 16 * PT; e += 64) sT[e] = 0.0f;   // padding columns stay zero
@@ -579,7 +583,7 @@ __global__ void __launch_bounds__(64 * TP_NW) k_triprod_bf16(const Group *__rest
 #pragma unroll
   for (int c = 0; c < NCT; ++c) acc[c] = z4;
   wave_sync();
-  for (int t = wave; t < g.count; t += TP_NW) {
+  for (int t = SW ? 0 : wave; t < g.count; t += SW ? 1 : TP_NW) {
     const Term tm = terms[g.first + t];
     const int nda = tm.nda, ndc = tm.ndc;
     const int Pa = (nda + 31) & ~31, Pc = (ndc + 31) & ~31;
@@ -639,6 +643,7 @@ __global__ void __launch_bounds__(64 * TP_NW) k_triprod_bf16(const Group *__rest
     }
     wave_sync();   // the strip is rewritten by the next term
   }
+  if constexpr (!SW) {
   // sum the waves: waves 1.. park their tiles, wave 0 adds and writes
   __syncthreads();
   if (wave > 0) {
@@ -657,6 +662,7 @@ __global__ void __launch_bounds__(64 * TP_NW) k_triprod_bf16(const Group *__rest
     for (int c = 0; c < NCT; ++c)
 #pragma unroll
       for (int v = 0; v < 4; ++v) acc[c][v] += red[(c * 4 + v) * 64 + lane];
+  }
   }
   double *O = obase + (size_t)w * ostride + (size_t)g.out_slot * nxy + g.out_off;
 #pragma unroll
