@@ -2389,7 +2389,8 @@ int stage_hamiltonian_fac(pnfam_gpu_ctx *h, int na, const int *act, int n3) {
   D.isstart = h->d_isstart.as<int>(); D.db = h->d_db.as<int>(); D.nsu = h->d_nsu.as<int>();
   D.nghl = h->nghl; D.RP = h->RP; D.dens = h->d_dens.as<double>(); D.act = act; D.n3 = n3;
   // few points: the pairs of a grid tile spread over psplit workgroups (psplit * na <= FAC_DPART)
-  D.psplit = std::max(1, std::min(8, FAC_DPART / na));
+  // (at most four parts: 2, 4, 8 and 16 measured 280, 254, 264 and 264 us for one point)
+  D.psplit = std::max(1, std::min(4, FAC_DPART / na));
   if (D.psplit > 1) {
     if (int e = h->d_fac_dpart.alloc((size_t)FAC_DPART * 2 * 2 * NDENS * h->nghl * 4)) return e;
     D.part = h->d_fac_dpart.as<float>();
