@@ -505,14 +505,17 @@ struct CombArgs {
   const int *act;
 };
 
-// One thread per (grid point, frequency slot, sector); threads of a block run
-// along (slot, sector), which is the contiguous direction of Of and Ok.
-__global__ void __launch_bounds__(128) k_mix_dens(CombArgs A) {
-  const int r = blockIdx.x;
-  const int j = blockIdx.y * 128 + threadIdx.x;
-  if (j >= 2 * A.na) return;
+// One thread per (grid point, frequency slot, sector) of a 16 x 16 tile; the
+// sixteen threads of a row run along (slot, sector), the contiguous direction
+// of Of and Ok, and the results go out through LDS so that sixteen threads
+// write sixteen consecutive grid points of one density.
+__global__ void __launch_bounds__(256) k_mix_dens(CombArgs A) {
+  __shared__ double sT[16][17];
+  const int tj = threadIdx.x & 15, tr = threadIdx.x >> 4;
+  const int r0 = blockIdx.x * 16, j0 = blockIdx.y * 16;
+  const int r = min(r0 + tr, A.nghl - 1);            // clamped: out-of-range rows are not written
+  const int j = min(j0 + tj, 2 * A.na - 1);
   const int slot = j >> 1, sec = j & 1;
-  const int w = A.act[slot];
   double acc[2 * NDENS];
 #pragma unroll
   for (int d = 0; d < 2 * NDENS; ++d) acc[d] = 0.0;
@@ -573,7 +576,16 @@ __global__ void __launch_bounds__(128) k_mix_dens(CombArgs A) {
     acc[2 * D_SBP] += pi; acc[2 * D_SBP + 1] -= pr;
   }
   const double wc = A.wdcori[r];
-  double *out = A.dens + (((size_t)w * 2 + sec) * 2 * NDENS) * A.nghl + r;
+  // writer view of the tile: wj = (slot, sector) row, wr = grid point
+  const int wj = threadIdx.x >> 4, wr = threadIdx.x & 15;
+  const int jw = j0 + wj, rw = r0 + wr;
+  const bool wok = jw < 2 * A.na && rw < A.nghl;
+  double *out = wok ? A.dens + (((size_t)A.act[jw >> 1] * 2 + (jw & 1)) * 2 * NDENS) * A.nghl + rw : nullptr;
 #pragma unroll
-  for (int d = 0; d < 2 * NDENS; ++d) out[(size_t)d * A.nghl] = acc[d] * wc;
+  for (int d = 0; d < 2 * NDENS; ++d) {
+    sT[tj][tr] = acc[d] * wc;
+    __syncthreads();
+    if (wok) out[(size_t)d * A.nghl] = sT[wj][wr];
+    __syncthreads();
+  }
 }

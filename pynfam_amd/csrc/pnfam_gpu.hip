@@ -2200,7 +2200,7 @@ int stage_hamiltonian_mixed(pnfam_gpu_ctx *h, int na, const int *act, int n3) {
     A.Of = h->mix[0].O.as<float>(); A.Ok0 = h->mix[1].O.as<float>(); A.Ok1 = h->mix[2].O.as<float>();
     A.ldf = 4L * cap; A.ldk = 2L * cap; A.RP = (int)RP; A.nghl = h->nghl; A.na = na;
     A.wdcori = h->d_wdcori.as<double>(); A.dens = h->d_dens.as<double>(); A.act = act;
-    hipLaunchKernelGGL(k_mix_dens, dim3(h->nghl, (2 * na + 127) / 128), dim3(128), 0, h->stream, A); }
+    hipLaunchKernelGGL(k_mix_dens, dim3((h->nghl + 15) / 16, (2 * na + 15) / 16), dim3(256), 0, h->stream, A); }
   // 4. fields (FP64, closed forms)
   if (int e = stage_fields(h, na, act)) return e;
   // 5. fields -> BF16 hi/lo
