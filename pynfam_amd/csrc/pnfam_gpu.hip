@@ -1026,7 +1026,7 @@ constexpr int BRO_MAXM = 64;
 struct BroArgs {
   double *vin, *vout;     // [w][N]
   void *df, *dv;          // history [w][M][N]: double, or float on the mixed path
-  double *pdf, *pdv;      // float history: the pending raw vout and vin [w][N] (FP64)
+  double *pdf;            // float history: the pending raw vout [w][N] (FP64)
   double *part;           // [w][nwg][2*M+2]
   double *gram;           // [w][M][M]
   double *coef;           // [w][M] gamma
@@ -1066,7 +1066,8 @@ __device__ inline double wg_max(double v, double *red) {
 // vout <- vout - vin; partial max|vout|; for iter >= 2 also finalise the history
 // difference at slot ipos and its partial squared norm.
 // H: element type of the history (double; float on the mixed path, whose
-// pending raw vout / vin stay FP64 in pdf / pdv so the differences are exact).
+// pending raw vout stays FP64 in pdf, so the residual difference is formed
+// exactly, and whose dv is the step the last update wrote to dvin).
 template <class H>
 __global__ void __launch_bounds__(BRO_WG) k_bro_resid(BroArgs B) {
   constexpr bool f64 = sizeof(H) == 8;
@@ -1079,7 +1080,9 @@ __global__ void __launch_bounds__(BRO_WG) k_bro_resid(BroArgs B) {
   H *dfp = (H *)B.df + ((size_t)w * B.M + (hist ? B.ipos : 0)) * N;
   H *dvp = (H *)B.dv + ((size_t)w * B.M + (hist ? B.ipos : 0)) * N;
   const double *qf = f64 ? nullptr : B.pdf + (size_t)w * N;
-  const double *qv = f64 ? nullptr : B.pdv + (size_t)w * N;
+  // float history (mixed path): the new dv is the step the last update added,
+  // which that update left in dvin
+  const double *qv = f64 ? nullptr : B.dvin + (size_t)w * N;
   double mx = 0.0, nr = 0.0, vm = 0.0;
   for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < N; e += (long)gridDim.x * blockDim.x) {
     const double d = vo[e] - vi[e];
@@ -1089,7 +1092,7 @@ __global__ void __launch_bounds__(BRO_WG) k_bro_resid(BroArgs B) {
     if (hist) {
       const double f = d - (f64 ? (double)dfp[e] : qf[e]);
       dfp[e] = (H)f;
-      dvp[e] = (H)(vi[e] - (f64 ? (double)dvp[e] : qv[e]));
+      dvp[e] = f64 ? (H)(vi[e] - (double)dvp[e]) : (H)qv[e];
       nr += f * f;
     }
   }
@@ -1303,7 +1306,6 @@ __global__ void __launch_bounds__(BRO_WG) k_bro_update(BroArgs B, int mode) {
         ((double *)B.dv)[on] = vi[e];
       } else {
         B.pdf[(size_t)w * N + e] = r;
-        B.pdv[(size_t)w * N + e] = vi[e];
       }
     }
     vi[e] += c;
@@ -1375,7 +1377,7 @@ struct pnfam_gpu_ctx {
   int cap = 0;
   int hist_cap = -1;
   DevPtr d_vin, d_vout, d_rsp, d_hsp, d_hqp, d_dens, d_flds, d_part, d_cpart, d_S, d_si, d_vmax, d_scal;
-  DevPtr d_df, d_dv, d_pdf, d_pdv, d_gram, d_coef, d_wre, d_wim, d_act;
+  DevPtr d_df, d_dv, d_pdf, d_gram, d_coef, d_wre, d_wim, d_act;
   // mixed path: the next increment of vin, accumulated FP64 H20/H02
   DevPtr d_act64, d_actlo, d_dvin, d_hacc;
   // split-BF16 grid stage: one element layout per block structure (the mean-field
@@ -1757,7 +1759,6 @@ int ensure_workspace(pnfam_gpu_ctx *h, int npts, int M) {
   if ((e = h->d_df.alloc(n * Mh * 4 * nxy * 8))) return e;
   if ((e = h->d_dv.alloc(n * Mh * 4 * nxy * 8))) return e;
   if ((e = h->d_pdf.alloc(n * 4 * nxy * 8))) return e;
-  if ((e = h->d_pdv.alloc(n * 4 * nxy * 8))) return e;
   if ((e = h->d_gram.alloc(n * Mh * Mh * 8))) return e;
   if ((e = h->d_coef.alloc(n * Mh * 8))) return e;
   if ((e = h->d_wre.alloc(n * 8))) return e;
@@ -3076,7 +3077,7 @@ int pnfam_gpu_solve_batch(pnfam_gpu_ctx *h, int32_t npts, const double *re_w, co
     // Broyden
     BroArgs B{};
     B.vin = h->d_vin.as<double>(); B.vout = h->d_vout.as<double>(); B.df = h->d_df.p; B.dv = h->d_dv.p;
-    B.pdf = h->d_pdf.as<double>(); B.pdv = h->d_pdv.as<double>();
+    B.pdf = h->d_pdf.as<double>();
     B.part = h->d_part.as<double>(); B.gram = h->d_gram.as<double>();
     B.coef = h->d_coef.as<double>(); B.si = h->d_si.as<double>(); B.scal = h->d_scal.as<double>();
     B.vmax = h->d_vmax.as<double>();
