@@ -552,29 +552,35 @@ __device__ inline void wave_sync() {
 // The grid is one-dimensional: workgroup L runs on XCD L % 8, and the strips of
 // one (group, point) unit get consecutive ids on one XCD, so they read the
 // unit's B operand through the same L2.
-// SW: the waves of a workgroup take four consecutive strips of one unit
-// instead, each summing every term itself; the four waves then read the same B
-// and C fragments at about the same time, mostly from the CU's L1.
-template <int NCT, bool SW>
+// SPW (strips per workgroup, 1, 2 or 4): the TP_NW waves take SPW consecutive
+// strips of one unit, TP_NW / SPW waves per strip splitting the terms.  With
+// more strips per workgroup the waves read the same B and C fragments together,
+// mostly from the CU's L1; with more waves per strip each wave's chain of terms
+// is shorter.  The waves of one strip are summed through LDS.
+template <int NCT, int SPW>
 __global__ void __launch_bounds__(64 * TP_NW) k_triprod_bf16(const Group *__restrict__ groups, int ngroups,
                                                              int nstrip, const Term *__restrict__ terms, TpMats M,
                                                              TpB BB, double *__restrict__ obase, long ostride,
                                                              const int *__restrict__ act, int na, long nxy, int n3) {
   constexpr int TW = (16 * NCT + 31) / 32 * 32;   // strip width padded to whole K steps
   constexpr int PT = TW + 4;
+  constexpr int WPS = TP_NW / SPW;                 // waves per strip
   __shared__ __attribute__((aligned(16))) float smem[TP_NW * 16 * PT];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int sub = wave % WPS;                      // this wave's share of the terms
   const int xcd = blockIdx.x & 7, loc = blockIdx.x >> 3;
-  const int nq = SW ? (nstrip + TP_NW - 1) / TP_NW : nstrip;   // workgroups per unit
-  const int strip = SW ? (loc % nq) * TP_NW + wave : loc % nq;
+  const int nq = (nstrip + SPW - 1) / SPW;         // workgroups per unit
+  const int strip = (loc % nq) * SPW + wave / WPS;
   const int unit = (loc / nq) * 8 + xcd;
   if (unit >= ngroups * na) return;                // uniform per workgroup
   const int gi = unit % ngroups, slot = unit / ngroups;
   const Group g = groups[gi];
   const int w = act[slot];
   const bool three = slot < n3;
+  if ((loc % nq) * SPW * 16 >= g.nrow) return;     // uniform per workgroup: no strip of it is live
   const int i0 = strip * 16;
-  if (i0 >= g.nrow) return;   // uniform per workgroup; per wave with SW, which has no block barriers
+  const bool live = i0 < g.nrow;                   // the waves of a dead strip still meet the barriers
+  if (WPS == 1 && !live) return;                   // one wave per strip: no block barriers
   const int fi = lane & 15, fg = lane >> 4;
   float *sT = smem + wave * 16 * PT;
   for (int e = lane; e < 16 * PT; e += 64) sT[e] = 0.0f;   // padding columns stay zero
@@ -583,7 +589,7 @@ __global__ void __launch_bounds__(64 * TP_NW) k_triprod_bf16(const Group *__rest
 #pragma unroll
   for (int c = 0; c < NCT; ++c) acc[c] = z4;
   wave_sync();
-  for (int t = SW ? 0 : wave; t < g.count; t += SW ? 1 : TP_NW) {
+  for (int t = sub; live && t < g.count; t += WPS) {
     const Term tm = terms[g.first + t];
     const int nda = tm.nda, ndc = tm.ndc;
     const int Pa = (nda + 31) & ~31, Pc = (ndc + 31) & ~31;
@@ -643,21 +649,22 @@ __global__ void __launch_bounds__(64 * TP_NW) k_triprod_bf16(const Group *__rest
     }
     wave_sync();   // the strip is rewritten by the next term
   }
-  if constexpr (!SW) {
-  // sum the waves: waves 1.. park their tiles, wave 0 adds and writes
+  if constexpr (WPS > 1) {
+  // sum the waves of each strip: waves 1.. of a strip park their tiles in the
+  // strip's slots, its first wave adds them in order and writes
   __syncthreads();
-  if (wave > 0) {
-    float *red = smem + (size_t)(wave - 1) * NCT * 4 * 64;
+  if (sub > 0) {
+    float *red = smem + (size_t)((wave / WPS) * (WPS - 1) + sub - 1) * NCT * 4 * 64;
 #pragma unroll
     for (int c = 0; c < NCT; ++c)
 #pragma unroll
       for (int v = 0; v < 4; ++v) red[(c * 4 + v) * 64 + lane] = acc[c][v];
   }
   __syncthreads();
-  if (wave != 0) return;
+  if (sub != 0 || !live) return;
 #pragma unroll
-  for (int j = 0; j < TP_NW - 1; ++j) {
-    const float *red = smem + (size_t)j * NCT * 4 * 64;
+  for (int j = 0; j < WPS - 1; ++j) {
+    const float *red = smem + (size_t)((wave / WPS) * (WPS - 1) + j) * NCT * 4 * 64;
 #pragma unroll
     for (int c = 0; c < NCT; ++c)
 #pragma unroll

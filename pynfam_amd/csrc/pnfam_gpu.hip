@@ -1702,21 +1702,22 @@ int launch_triprod_bf16(pnfam_gpu_ctx *h, int stage, const double *bbase, long b
     const int n = merge ? ntot : h->tpa_count[stage][c];
     if (!n) continue;
     const int ns = merge ? smax : h->tpa_strips[stage][c];
-    // qp -> sp classes of four or more strips, many points: one strip per wave
-    // (shared B and C loads; measured 17 % faster for that stage).  The sp -> qp
-    // groups hold more terms, whose serial chain per wave made that stage 13 %
-    // slower, and merged few-point launches want the parallelism of the term split.
-    const bool sw = stage == 0 && ns >= TP_NW && !merge;
+    // strips per workgroup (one launch per tile class with many points): qp -> sp
+    // classes of four or more strips take one strip per wave (shared B and C
+    // loads; 17 % faster for that stage).  The sp -> qp groups hold more terms:
+    // two and four strips per workgroup measured 5 % and 10 % slower there, and
+    // merged few-point launches want the parallelism of the term split.
+    const int spw = (!merge && stage == 0 && ns >= 4) ? 4 : 1;
     const long units8 = ((long)n * na + 7) / 8;
-    const dim3 grid((unsigned)(units8 * (sw ? (ns + TP_NW - 1) / TP_NW : ns) * 8));
+    const dim3 grid((unsigned)(units8 * ((ns + spw - 1) / spw) * 8));
     const Group *g = groups + (merge ? 0 : h->tpa_first[stage][c]);
 #define TB_LAUNCH(NC)                                                                                          \
   do {                                                                                                         \
-    if (sw)                                                                                                    \
-      hipLaunchKernelGGL((k_triprod_bf16<NC, true>), grid, dim3(64 * TP_NW), 0, h->stream, g, n, ns, terms, M, BB, \
+    if (spw == 4)                                                                                              \
+      hipLaunchKernelGGL((k_triprod_bf16<NC, 4>), grid, dim3(64 * TP_NW), 0, h->stream, g, n, ns, terms, M, BB, \
                          obase, ostride, act, na, h->nxy, n3);                                                 \
     else                                                                                                       \
-      hipLaunchKernelGGL((k_triprod_bf16<NC, false>), grid, dim3(64 * TP_NW), 0, h->stream, g, n, ns, terms, M, BB, \
+      hipLaunchKernelGGL((k_triprod_bf16<NC, 1>), grid, dim3(64 * TP_NW), 0, h->stream, g, n, ns, terms, M, BB, \
                          obase, ostride, act, na, h->nxy, n3);                                                 \
   } while (0)
     switch (c) {
