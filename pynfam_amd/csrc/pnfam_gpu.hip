@@ -1040,6 +1040,14 @@ struct BroArgs {
   const int *act;
 };
 
+// Partial sums of the Broyden reductions, value-major so that the readers'
+// lanes load consecutive workgroups: part[(w * (2 BRO_MAXM + 2) + i) * nwg + q]
+// for workgroup q; i = 0 max|vout|, 1 |df|^2, 2 max|vin| (resid, norm), then
+// 2 + 2 j, 3 + 2 j the dots of history entry j (dots, solve; slot 2 is reused).
+__device__ inline double *bro_part(const BroArgs &B, int w, int i, int q) {
+  return B.part + ((size_t)w * (2 * BRO_MAXM + 2) + i) * B.nwg + q;
+}
+
 __device__ inline double wg_sum(double v, double *red) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
   const int lane = threadIdx.x % 64, wid = threadIdx.x / 64;
@@ -1100,10 +1108,9 @@ __global__ void __launch_bounds__(BRO_WG) k_bro_resid(BroArgs B) {
   const double s = wg_sum(nr, red);
   const double v = wg_max(vm, red);
   if (threadIdx.x == 0) {
-    double *pp = B.part + ((size_t)w * B.nwg + blockIdx.x) * (2 * BRO_MAXM + 2);
-    pp[0] = m;
-    pp[1] = s;
-    pp[2] = v;   // read by k_bro_norm before k_bro_dots reuses the slot
+    *bro_part(B, w, 0, blockIdx.x) = m;
+    *bro_part(B, w, 1, blockIdx.x) = s;
+    *bro_part(B, w, 2, blockIdx.x) = v;   // read by k_bro_norm before k_bro_dots reuses the slot
   }
 }
 
@@ -1113,10 +1120,9 @@ __global__ void __launch_bounds__(64) k_bro_norm(BroArgs B) {
   const int w = B.act[blockIdx.x];
   double m = 0.0, s = 0.0, v = 0.0;
   for (int k = threadIdx.x; k < B.nwg; k += 64) {
-    const double *pp = B.part + ((size_t)w * B.nwg + k) * (2 * BRO_MAXM + 2);
-    m = fmax(m, pp[0]);
-    s += pp[1];
-    v = fmax(v, pp[2]);
+    m = fmax(m, *bro_part(B, w, 0, k));
+    s += *bro_part(B, w, 1, k);
+    v = fmax(v, *bro_part(B, w, 2, k));
   }
   for (int o = 32; o > 0; o >>= 1) {
     m = fmax(m, __shfl_xor(m, o, 64));
@@ -1130,7 +1136,7 @@ __global__ void __launch_bounds__(64) k_bro_norm(BroArgs B) {
 }
 
 // scale the history slot, then partial dots df_j . df_ipos and df_j . vout,
-// eight history vectors per pass over the new difference and vout
+// BRO_JB history vectors per pass over the new difference and vout
 constexpr int BRO_JB = 8;
 template <class H>
 __global__ void __launch_bounds__(BRO_WG) k_bro_dots(BroArgs B) {
@@ -1144,7 +1150,7 @@ __global__ void __launch_bounds__(BRO_WG) k_bro_dots(BroArgs B) {
   const long e0 = (long)blockIdx.x * blockDim.x + threadIdx.x, es = (long)gridDim.x * blockDim.x;
   if (B.iter >= 2)
     for (long e = e0; e < N; e += es) { dfp[e] = (H)(dfp[e] * normi); dvp[e] = (H)(dvp[e] * normi); }
-  double *pp = B.part + ((size_t)w * B.nwg + blockIdx.x) * (2 * BRO_MAXM + 2) + 2;
+
   for (int j0 = 0; j0 < B.iter_used; j0 += BRO_JB) {
     const int nj = min(BRO_JB, B.iter_used - j0);
     double g[BRO_JB], v[BRO_JB];
@@ -1179,7 +1185,7 @@ __global__ void __launch_bounds__(BRO_WG) k_bro_dots(BroArgs B) {
     if (threadIdx.x < 2 * nj) {
       double t = 0.0;
       for (int q = 0; q < BRO_WG / 64; ++q) t += red[threadIdx.x * (BRO_WG / 64) + q];
-      pp[2 * j0 + threadIdx.x] = t;
+      *bro_part(B, w, 2 + 2 * j0 + threadIdx.x, blockIdx.x) = t;
     }
   }
 }
@@ -1200,9 +1206,8 @@ __global__ void __launch_bounds__(256) k_bro_solve(BroArgs B) {
     for (int j = wv; j < k; j += nwv) {
       double sg = 0.0, sv = 0.0;
       for (int q = lane; q < B.nwg; q += 64) {
-        const double *pp = B.part + ((size_t)w * B.nwg + q) * (2 * BRO_MAXM + 2) + 2;
-        sg += pp[2 * j];
-        sv += pp[2 * j + 1];
+        sg += *bro_part(B, w, 2 + 2 * j, q);
+        sv += *bro_part(B, w, 3 + 2 * j, q);
       }
       for (int o = 32; o > 0; o >>= 1) {
         sg += __shfl_xor(sg, o, 64);
