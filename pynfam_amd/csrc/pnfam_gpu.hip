@@ -1221,13 +1221,15 @@ __global__ void __launch_bounds__(256) k_bro_solve(BroArgs B) {
   }
   __syncthreads();
   const double w0 = 0.010;
-  for (int e = threadIdx.x; e < k * 2 * k; e += blockDim.x) {
-    const int i = e / (2 * k), j = e % (2 * k);
-    double v;
-    if (j < k) v = (i == j) ? w0 * w0 + 1.0 : gram[(size_t)i * M + j];
-    else v = (j - k == i) ? 1.0 : 0.0;
-    a[i][j] = v;
-  }
+  // rows over the waves, columns over the lanes (no integer division per element)
+  const int gl = threadIdx.x & 63, gw = threadIdx.x >> 6, gnw = blockDim.x >> 6;
+  for (int i = gw; i < k; i += gnw)
+    for (int j = gl; j < 2 * k; j += 64) {
+      double v;
+      if (j < k) v = (i == j) ? w0 * w0 + 1.0 : gram[(size_t)i * M + j];
+      else v = (j - k == i) ? 1.0 : 0.0;
+      a[i][j] = v;
+    }
   __syncthreads();
   // Gauss-Jordan with partial pivoting (the first largest |a[i][c]|), three
   // barriers per column: wave 0 finds the pivot; the rows are swapped, row c
@@ -1265,9 +1267,10 @@ __global__ void __launch_bounds__(256) k_bro_solve(BroArgs B) {
       a[c][j] = pv * d;
     }
     __syncthreads();
-    for (int e = threadIdx.x; e < k * 2 * k; e += blockDim.x) {
-      const int i = e / (2 * k), j = e % (2 * k);
-      if (i != c) a[i][j] -= fac[i] * a[c][j];
+    for (int i = gw; i < k; i += gnw) {
+      if (i == c) continue;
+      const double f = fac[i];
+      for (int j = gl; j < 2 * k; j += 64) a[i][j] -= f * a[c][j];
     }
     __syncthreads();
   }
