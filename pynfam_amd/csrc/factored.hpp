@@ -12,12 +12,17 @@
 //
 // Work per frequency is that of the FP64 kernels: no pair tables are streamed,
 // so a pass over few frequencies costs little.  The omega-independent psi
-// operands are BF16 hi/lo tables built once per operator:
-//   PA[z][state][RP]             projection rows (k = grid point, contiguous)
-//   DB[x][chunk][RP][32]         density columns: chunks of 32 states of one
-//                                spin of one block, zero-padded
+// operands are tables built once per operator:
+//   PA[z][state][RP]             BF16 hi/lo, projection rows (k = grid point, contiguous)
+//   PS[z][state][RP]             FP32, the vector work (B of the projection, psi_y of
+//                                the densities)
+//   DB[x][chunk][RP][32]         BF16 hi/lo, density columns: chunks of 32 states of
+//                                one spin of one block, zero-padded
 // The per-frequency rho/kappa operand is converted once per pass into the same
-// chunking (FR, one 32-state chunk per (pair, column b, spin of a, chunk)).
+// chunking (FR, one 32-state chunk per (pair, column b, spin of a, chunk)), and
+// the fields into the order the projection stages them (FFm, FFp).
+// The same file holds the split-BF16 block transforms (k_triprod_bf16) of the
+// mixed path.
 //
 // MFMA operand layout (16x16x32, wave64): lane i + 16 g supplies A[i][8g..8g+7]
 // and B[8g..8g+7][i]; the result D[4 g + v][i] sits in lane i + 16 g, register v.
