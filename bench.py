@@ -15,7 +15,11 @@ The JSON line also carries:
   against the FP64 path on the same contour;
 - the dominant kernel region's achieved rate against the MI355X peak of the
   arithmetic it runs (HIP events around every launch on the engine stream
-  during the timed steps);
+  during the timed steps): `achieved` counts the algorithmic work (for the grid
+  stage the reference's 88 nghl nxy FLOP per point and iteration, half each to
+  the density and the projection regions, once per point whatever the number of
+  split products), `mfma_issued_tflops` what the region's matrix instructions
+  issue (pair-table GEMMs, every split product);
 - the CPU oracle timed on a bounded sample of the same workload on this host.
 """
 import argparse
@@ -137,12 +141,22 @@ def measure(engines, w, precision, args):
         pool.shutdown()
     prof = {}
     for e in engines:
-        for k, v in e.profile_read().items():
-            d = prof.setdefault(k, {"ms": 0.0, "launches": 0, "units": 0.0, "flops": 0.0})
+        pe = e.profile_read()
+        # algorithmic FLOPs of the grid stage (the reference's count per point and
+        # iteration, 88 nghl nxy: density 44 + projection 44) for the regions that
+        # run it; the points of a pass are the units of its one-per-pass kernel
+        grid_half = 44.0 * e.prob.basis.nghl * e.nxy
+        pts = {"mix_gemm_density": pe.get("mix_densities", {}).get("units", 0.0),
+               "mix_gemm_project": pe.get("mix_densities", {}).get("units", 0.0),
+               "fac_density": pe.get("fac_convert", {}).get("units", 0.0),
+               "fac_project": pe.get("fac_convert", {}).get("units", 0.0)}
+        for k, v in pe.items():
+            d = prof.setdefault(k, {"ms": 0.0, "launches": 0, "units": 0.0, "flops": 0.0, "alg": 0.0})
             d["ms"] += v["ms"]
             d["launches"] += v["launches"]
             d["units"] += v["units"]
             d["flops"] += v["flops_per_unit"] * v["units"]
+            d["alg"] += grid_half * pts[k] if k in pts else v["flops_per_unit"] * v["units"]
         e.profile(False)
     flop_kernels = {k: v for k, v in prof.items() if v["flops"] > 0}
     return {"value": conv_total / dt, "ms_per_step": dt * 1e3 / args.steps, "converged": conv_total,
@@ -211,11 +225,14 @@ def main():
     head = runs[modes[-1]]
     name, d = head["dominant"]
     peak, unit = KERNEL_PEAK.get(name, (PEAK_FP64_TFLOPS, "TFLOP/s"))
-    achieved = d["flops"] / (d["ms"] * 1e-3) / 1e12 if d["ms"] > 0 else 0.0
+    sec = d["ms"] * 1e-3
+    achieved = d["alg"] / sec / 1e12 if d["ms"] > 0 else 0.0
     roofline = {"kernel": name, "bound": "mfma", "achieved": achieved, "peak": peak,
                 "unit": unit, "frac": achieved / peak, "traffic": measured_traffic(name),
                 "avg_launch_ms": d["ms"] / max(d["launches"], 1), "launches": d["launches"],
-                "algorithmic_flop_per_launch": d["flops"] / max(d["launches"], 1)}
+                "algorithmic_flop_per_launch": d["alg"] / max(d["launches"], 1),
+                "mfma_issued_flop_per_launch": d["flops"] / max(d["launches"], 1),
+                "mfma_issued_tflops": d["flops"] / sec / 1e12 if d["ms"] > 0 else 0.0}
     cpu = None
     if args.cpu_iters > 0:
         cpu = cpu_baseline(probs[0], runs[modes[0]]["mean_iterations"], args.cpu_iters, args.cpu_threads)

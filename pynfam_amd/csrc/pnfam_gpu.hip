@@ -2419,7 +2419,7 @@ int stage_hamiltonian_fac(pnfam_gpu_ctx *h, int na, const int *act, int n3) {
   P.FFm = h->FFm.as<float>(); P.FFp = h->FFp.as<float>();
   P.hsp = h->d_hsp.as<double>(); P.nxy = h->nxy; P.nghl = h->nghl; P.RP = h->RP;
   P.act = act; P.n3 = n3;
-  { Prof pr(h, K_FAC_CVT, na);
+  { Prof pr(h, K_FAC_CVT, 0);   // its units (points) are counted once per pass, with k_fac_rho
     const int nconv = h->RP * (FAC_FM + FAC_FP);
     hipLaunchKernelGGL(k_fac_fconv, dim3((nconv + 255) / 256, na, 4), dim3(256), 0, h->stream, h->d_flds.as<double>(),
                        h->nghl, h->RP, act, h->FFm.as<float>(), h->FFp.as<float>()); }
