@@ -168,12 +168,12 @@ def measure(engines, w, precision, args):
             "strength": S}
 
 
-def rate_error(emax, npts, s_ref, s_mix):
+def rate_error(emax, npts, s_ref, s_mix, z_parent=Z_PARENT, n_parent=N_PARENT):
     """Allowed-GT beta-minus rates (K=0, K=1 and total) of the daughter
     Z+1, A from the FP64 and the mixed strengths on the same contour and phase
     space; relative differences."""
     from pynfam_amd.rate import gt_rates_from_half
-    zd, a = Z_PARENT + 1, Z_PARENT + N_PARENT
+    zd, a = z_parent + 1, z_parent + n_parent
     r64, _, _ = gt_rates_from_half(zd, a, emax, 0.0, 2 * npts, {0: s_ref[:npts], 1: s_ref[npts:]})
     rmx, _, _ = gt_rates_from_half(zd, a, emax, 0.0, 2 * npts, {0: s_mix[:npts], 1: s_mix[npts:]})
     return {"total": abs(rmx["total"] - r64["total"]) / abs(r64["total"]),
@@ -190,6 +190,8 @@ def main():
     ap.add_argument("--nsh", type=int, default=16)
     ap.add_argument("--beta", type=float, default=0.3)
     ap.add_argument("--ngrid", type=int, default=40)
+    ap.add_argument("--z", type=int, default=Z_PARENT, help="protons of the parent nucleus")
+    ap.add_argument("--n", type=int, default=N_PARENT, help="neutrons of the parent nucleus")
     ap.add_argument("--npts", type=int, default=128, help="frequencies per K per step")
     ap.add_argument("--max-iter", type=int, default=300)
     ap.add_argument("--cpu-iters", type=int, default=10, help="oracle iterations in the CPU sample (0: skip)")
@@ -213,7 +215,7 @@ def main():
     from pynfam_amd import engine as E
 
     from pynfam_amd.rate import eqrpa_max_beta_minus
-    basis, probs = build_workload(args.nsh, args.beta, Z_PARENT, N_PARENT, args.ngrid)
+    basis, probs = build_workload(args.nsh, args.beta, args.z, args.n, args.ngrid)
     emax = eqrpa_max_beta_minus(basis.sols[0].ala, basis.sols[1].ala)
     engines = [E.GpuEngine(p) for p in probs]
     w = contour(args.npts, emax)
@@ -243,7 +245,7 @@ def main():
            "scaling": "weak", "vs_baseline": None,
            "dtype": "f64" if modes[-1] == E.FP64 else "bf16x3/fp32-acc grid stage, f64 elsewhere",
            "data": "synthetic",
-           "config": {"workload": f"pnFAM GT- K=0,1, synthetic HFB Z=60 N=92 beta2={args.beta} N_sh={args.nsh} "
+           "config": {"workload": f"pnFAM GT- K=0,1, synthetic HFB Z={args.z} N={args.n} beta2={args.beta} N_sh={args.nsh} "
                                   f"{args.ngrid}x{args.ngrid} grid, {len(w)} contour points per K per step, "
                                   f"{mixed_name[modes[-1]]}, tol 1e-7, Broyden M=50",
                       "dqp": int(basis.dqp), "nxy_K0": engines[0].nxy, "nxy_K1": engines[1].nxy,
@@ -258,7 +260,7 @@ def main():
                        "mean_iterations": ref["mean_iterations"], "kernel_ms": ref["kernel_ms"]}
         out["mixed_vs_fp64"] = mix["value"] / ref["value"]
         out["strength_rel_err_vs_fp64"] = {"max": float(err.max()), "median": float(np.median(err))}
-        out["rate_rel_err_vs_fp64"] = rate_error(emax, len(w), ref["strength"], mix["strength"])
+        out["rate_rel_err_vs_fp64"] = rate_error(emax, len(w), ref["strength"], mix["strength"], args.z, args.n)
     print(json.dumps(out), flush=True)
 
 
