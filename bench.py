@@ -198,7 +198,7 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--concurrent", type=int, default=1,
                     help="solve the K=0 and K=1 batches from two host threads at once (1) or in turn (0)")
-    ap.add_argument("--precision", choices=["fp64", "mixed", "mixed_refine", "both"], default="both",
+    ap.add_argument("--precision", choices=["fp64", "mixed", "mixed_refine", "both", "both_refine"], default="both",
                     help="both: FP64 then mixed; the JSON value is the last one")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB (calibrates FETCH_SIZE/WRITE_SIZE)")
@@ -220,7 +220,7 @@ def main():
     engines = [E.GpuEngine(p) for p in probs]
     w = contour(args.npts, emax)
     modes = {"fp64": [E.FP64], "mixed": [E.MIXED_NOREFINE], "mixed_refine": [E.MIXED],
-             "both": [E.FP64, E.MIXED_NOREFINE]}[args.precision]
+             "both": [E.FP64, E.MIXED_NOREFINE], "both_refine": [E.FP64, E.MIXED]}[args.precision]
     runs = {}
     for prec in modes:
         runs[prec] = measure(engines, w, prec, args)
