@@ -1021,6 +1021,8 @@ __global__ void k_hacc(double *__restrict__ hacc, const double *__restrict__ hqp
 
 // ---- Broyden (pnfam_broyden.f90:117-216), per frequency, FP64 ---------------
 constexpr int BRO_WG = 256;
+constexpr int BRO_NWG_FEW = 1024;   // workgroups per point of the FP32-history kernels with few points
+constexpr int BRO_FEW = 8;          // "few": at most this many active points
 constexpr int BRO_MAXM = 64;
 
 struct BroArgs {
@@ -1439,6 +1441,7 @@ struct pnfam_gpu_ctx {
   double *pin_res = nullptr;
   int pin_cap = 0;
   int bro_nwg = 0;
+  int bro_nwg_few = 0;          // FP32-history Broyden with few active points: more workgroups per point
   double flops_q2s = 0, flops_s2q = 0;
   // per-kernel event timing (pnfam_gpu_profile)
   bool prof = false;
@@ -1759,7 +1762,8 @@ int ensure_workspace(pnfam_gpu_ctx *h, int npts, int M) {
   if ((e = h->d_dens.alloc(n * 2 * 2 * NDENS * (size_t)h->nghl * 8))) return e;
   if ((e = h->d_flds.alloc(n * 2 * 2 * NFLD * (size_t)h->nghl * 8))) return e;
   h->bro_nwg = (int)std::min<long>(256, std::max<long>(1, (long)(4 * nxy + BRO_WG - 1) / BRO_WG));
-  if ((e = h->d_part.alloc(n * h->bro_nwg * (2 * BRO_MAXM + 2) * 8))) return e;
+  h->bro_nwg_few = (int)std::min<long>(BRO_NWG_FEW, std::max<long>(1, (long)(4 * nxy + BRO_WG - 1) / BRO_WG));
+  if ((e = h->d_part.alloc(n * std::max(h->bro_nwg, h->bro_nwg_few) * (2 * BRO_MAXM + 2) * 8))) return e;
   if ((e = h->d_cpart.alloc(n * 64 * 2 * 8))) return e;
   if ((e = h->d_S.alloc(n * 2 * 8))) return e;
   if ((e = h->d_si.alloc(n * 8))) return e;
@@ -3091,7 +3095,10 @@ int pnfam_gpu_solve_batch(pnfam_gpu_ctx *h, int32_t npts, const double *re_w, co
     B.coef = h->d_coef.as<double>(); B.si = h->d_si.as<double>(); B.scal = h->d_scal.as<double>();
     B.vmax = h->d_vmax.as<double>();
     B.dvin = mixed ? h->d_dvin.as<double>() : nullptr;
-    B.N = N; B.M = std::max(M, 0); B.nwg = nwg; B.iter = iter; B.alpha = o->alpha_mix; B.act = dact;
+    // with the FP32 history (its reductions need not match the FP64 path's order)
+    // and few points, more workgroups per point keep more loads in flight
+    const int nwg_it = (hist32 && na <= BRO_FEW) ? h->bro_nwg_few : nwg;
+    B.N = N; B.M = std::max(M, 0); B.nwg = nwg_it; B.iter = iter; B.alpha = o->alpha_mix; B.act = dact;
     int mode = 0;
     if (M < 0) mode = 2;
     else if (M == 0 || iter == 0) mode = 1;
@@ -3103,17 +3110,17 @@ int pnfam_gpu_solve_batch(pnfam_gpu_ctx *h, int32_t npts, const double *re_w, co
     }
     B.iter = mode == 0 ? iter : 0;
     { Prof pr(h, K_BRO_RESID, na);
-      if (hist32) hipLaunchKernelGGL(k_bro_resid<float>, dim3(nwg, na), dim3(BRO_WG), 0, st, B);
+      if (hist32) hipLaunchKernelGGL(k_bro_resid<float>, dim3(nwg_it, na), dim3(BRO_WG), 0, st, B);
       else hipLaunchKernelGGL(k_bro_resid<double>, dim3(nwg, na), dim3(BRO_WG), 0, st, B); }
     { Prof pr(h, K_BRO_NORM, na); hipLaunchKernelGGL(k_bro_norm, dim3(na), dim3(64), 0, st, B); }
     if (mode == 0) {
       { Prof pr(h, K_BRO_DOTS, na);
-        if (hist32) hipLaunchKernelGGL(k_bro_dots<float>, dim3(nwg, na), dim3(BRO_WG), 0, st, B);
+        if (hist32) hipLaunchKernelGGL(k_bro_dots<float>, dim3(nwg_it, na), dim3(BRO_WG), 0, st, B);
         else hipLaunchKernelGGL(k_bro_dots<double>, dim3(nwg, na), dim3(BRO_WG), 0, st, B); }
       { Prof pr(h, K_BRO_SOLVE, na); hipLaunchKernelGGL(k_bro_solve, dim3(na), dim3(256), 0, st, B); }
     }
     { Prof pr(h, K_BRO_UPDATE, na);
-      if (hist32) hipLaunchKernelGGL(k_bro_update<float>, dim3(nwg, na), dim3(BRO_WG), 0, st, B, mode);
+      if (hist32) hipLaunchKernelGGL(k_bro_update<float>, dim3(nwg_it, na), dim3(BRO_WG), 0, st, B, mode);
       else hipLaunchKernelGGL(k_bro_update<double>, dim3(nwg, na), dim3(BRO_WG), 0, st, B, mode); }
     const int cnwg = 64;
     { Prof pr(h, K_CONTRACT, na);
