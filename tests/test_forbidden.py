@@ -85,3 +85,41 @@ def test_gpu_strengths_and_crossterms(op, K):
         want = np.array([complex(*p["crossterms"][lbl]) for p in pts])
         assert np.max(np.abs(got - want) / np.maximum(np.abs(want), 1e-3 * np.abs(S))) < 1e-7, lbl
     eng.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode,grid", [("MIXED", "auto"), ("MIXED_NOREFINE", "tables"),
+                                       ("MIXED_NOREFINE", "factored")])
+@pytest.mark.parametrize("op,K", OPK)
+def test_gpu_mixed_strengths_and_crossterms(op, K, mode, grid, monkeypatch):
+    """The mixed path on every operator of the set: parity-changing (R, P, PS0)
+    and K=2 (RS2) operators give block pairings and spin classes that GT does
+    not.  Tolerances in S as for the GT runs: 1e-6 relative with the FP64
+    refinement, 1e-4 without.  A cross-term contracts the amplitudes with another
+    operator's F; its error follows the amplitudes' error, not its own size,
+    which is small where the two responses cancel (R x P: |G| = 8e-3 |S| at the
+    first point).  Its error is therefore taken relative to the largest |G| of
+    the point set: 1e-5 with the refinement, 1e-4 without."""
+    from pynfam_amd import engine as E
+    from pynfam_amd.crossterms import contract, crossterm_problems
+    tol = {"MIXED": 1e-6, "MIXED_NOREFINE": 1e-4}[mode]
+    tol_g = {"MIXED": 1e-5, "MIXED_NOREFINE": 1e-4}[mode]
+    monkeypatch.setenv("PNFAM_GPU_MIX_GRID", grid)
+    prob = all_problem(op, K)
+    pts = _points()[f"{op}-K{K}"]
+    w = np.array([complex(*p["energy"]) for p in pts])
+    eng = E.GpuEngine(prob)
+    r = eng.solve(w, precision=getattr(E, mode), want_x=True)
+    S = np.array([complex(*p["strength"]) for p in pts])
+    assert np.all(r["iterations"] > 0), r["iterations"]
+    err = np.max(np.abs(r["strength"] - S) / np.abs(S))
+    print(f"{op} K={K} {mode} {grid}: max rel err {err:.2e}, iterations {r['iterations'].tolist()}")
+    assert err < tol, err
+    for lbl, gp in crossterm_problems(prob):
+        ge = E.GpuEngine(gp)
+        g20, g02 = ge.fqp()
+        ge.close()
+        got = contract(r["x"], g20, g02)
+        want = np.array([complex(*p["crossterms"][lbl]) for p in pts])
+        assert np.max(np.abs(got - want)) / np.max(np.abs(want)) < tol_g, lbl
+    eng.close()
