@@ -76,10 +76,21 @@ def coulomb_l0(zd):
     return 0.5 * (1.0 + gamma_k(1, zd))
 
 
-def fermi_f0(zd, a, w):
-    """Point-charge Fermi function F0 of electron energy w (units of m_e c^2),
-    without L0.  Evaluated through its logarithm: exp(pi y) and |Gamma(g + iy)|^2
-    over- and underflow separately near threshold."""
+def _odd_factorial2(n):
+    """(2k-1)!! for odd n = 2k-1 >= 1."""
+    out = 1
+    for j in range(n, 0, -2):
+        out *= j
+    return out
+
+
+def fermi_fk(k, zd, a, w):
+    """Generalised point-charge Fermi function F_{k-1} of electron energy w
+    (units of m_e c^2), without L0 (phase_space.py:881-982):
+        (k (2k-1)!!)^2 4^k (2 p R)^(2 (gamma_k - k)) e^(pi y) |Gamma(gamma_k + i y)|^2
+        / Gamma(2 gamma_k + 1)^2,  y = alpha Z w / p.
+    Evaluated through its logarithm: exp(pi y) and |Gamma(g + iy)|^2 over- and
+    underflow separately near threshold.  Zero at and below w = 1."""
     w = np.asarray(w, dtype=float)
     out = np.zeros_like(w)
     ok = ~((np.abs(w - 1.0) < _EPS) | (w < 1.0))
@@ -88,22 +99,42 @@ def fermi_f0(zd, a, w):
     wk = w[ok]
     p = np.sqrt(wk * wk - 1.0)
     radius = R0 / HBAR_MEC * a ** (1.0 / 3.0)
-    g = gamma_k(1, zd)
+    g = gamma_k(k, zd)
     y = ALPHA * zd * wk / p
-    # 4 (2 p R)^(2 (g - 1)) e^(pi y) |Gamma(g + i y)|^2 / Gamma(2 g + 1)^2
     lg = loggamma(g + 1j * y)
-    ln_f = (np.log(4.0) + 2.0 * (g - 1.0) * np.log(2.0 * p * radius) + np.pi * y + 2.0 * lg.real
+    ln_f = (2.0 * np.log(float(k * _odd_factorial2(2 * k - 1))) + k * np.log(4.0)
+            + 2.0 * (g - k) * np.log(2.0 * p * radius) + np.pi * y + 2.0 * lg.real
             - 2.0 * loggamma(2.0 * g + 1.0).real)
     out[ok] = np.exp(ln_f)
     return out
 
 
+def fermi_f0(zd, a, w):
+    """Point-charge Fermi function F0 (k = 1 of fermi_fk)."""
+    return fermi_fk(1, zd, a, w)
+
+
+def lambda_2(zd, a, w):
+    """Coulomb function lambda_2 = (2 + gamma_2) / (2 (1 + gamma_1)) F1 / F0
+    (phase_space.py:785-826); zero where F0 vanishes."""
+    w = np.asarray(w, dtype=float)
+    f0, f1 = fermi_fk(1, zd, a, w), fermi_fk(2, zd, a, w)
+    out = np.zeros_like(w)
+    ok = f0 > 0.0
+    out[ok] = (2.0 + gamma_k(2, zd)) / (2.0 * (1.0 + gamma_k(1, zd))) * f1[ok] / f0[ok]
+    return out
+
+
 def psi_integrand(wx, n, zd, a, w0):
-    """Integrand of f_n for beta-minus at electron energy wx and endpoint w0."""
+    """Integrand of f_n (n = 1..6) for beta-minus at electron energy wx and
+    endpoint w0 (phase_space.py:253-313)."""
     wx = np.maximum(np.asarray(wx, dtype=float), 1.0)
     px = np.sqrt(wx * wx - 1.0)
     pv = w0 - wx
-    gn = {1: gamma_k(1, zd), 2: wx, 3: wx * wx, 4: wx ** 3, 5: wx * pv * pv}[n]
+    if n == 6:
+        gn = wx * px * px * lambda_2(zd, a, wx)
+    else:
+        gn = {1: gamma_k(1, zd), 2: wx, 3: wx * wx, 4: wx ** 3, 5: wx * pv * pv}[n]
     f = px * pv * pv * gn * fermi_f0(zd, a, wx) * coulomb_l0(zd)
     f[~np.isfinite(f)] = 0.0
     return f

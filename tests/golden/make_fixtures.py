@@ -229,6 +229,45 @@ def extract_rates(name, rel):
     print(name, "rates", sorted(rates))
 
 
+def extract_ff_rates(name, rel):
+    """The archived beta-decay post-processing of the all-operator run: the
+    contour table of every operator-K (strength and cross-term columns, weights,
+    dz/dtheta), f1..f6 and the shape factor of every rate channel on the contour,
+    the rates of beta.out and the ground-state transition energy E_gs of
+    logfile_mini.dat (the HFB Q value is EQRPA_max - E_gs)."""
+    src = os.path.join(REF, rel)
+    out = {"contour": {}}
+    for fn in sorted(os.listdir(os.path.join(src, "fam_soln"))):
+        if not fn.endswith(".out"):
+            continue
+        _, cols, rows = _table(os.path.join(src, "fam_soln", fn))
+        keep = [i for i, c in enumerate(cols) if c not in ("Time", "Conv")]
+        out["contour"][fn[:-4]] = {"columns": [cols[i] for i in keep], "rows": [[r[i] for i in keep] for r in rows]}
+    meta = os.path.join(src, "beta_soln", "beta_meta")
+    for part in ("re", "im"):
+        _, cols, rows = _table(os.path.join(meta, "phasespace_%s.out" % part))
+        out["phasespace_" + part] = {"columns": cols, "rows": rows}
+    _, cols, rows = _table(os.path.join(meta, "shapefactor_re.out"))
+    out["shapefactor_re"] = {"columns": cols, "rows": rows}
+    _, cols, rows = _table(os.path.join(src, "beta_soln", "shapefactor_im.out"))
+    out["shapefactor_im"] = {"columns": cols, "rows": rows}
+    head, _, _ = _table(os.path.join(src, "beta_soln", "beta.out"))
+    out["summary"] = _summary(head)
+    rates = {}
+    with open(os.path.join(src, "beta_soln", "beta.out")) as fh:
+        for l in fh.read().splitlines():
+            m = re.match(r"(\S+)\s+(" + FLOAT + r")\s+(\S+)$", l)
+            if m and not l.startswith("#"):
+                rates[m.group(1)] = _f(m.group(2))
+    out["rates"] = rates
+    _, cols, rows = _table(os.path.join(src, "beta_soln", "logfile_mini.dat"))
+    # the header names the label column; the row's leading index is dropped
+    out["E_gs"] = float(rows[0][cols.index("E_gs")])
+    with open(os.path.join(HERE, name, "rates.json"), "w") as fh:
+        json.dump(out, fh, indent=0)
+    print(name, "contour tables", sorted(out["contour"]), "E_gs", out["E_gs"])
+
+
 def extract_points_only(name, rel, every=6):
     """HFB input and every `every`-th archived point of each operator-K (the
     full traces are kept; cross-terms included)."""
@@ -271,6 +310,7 @@ def main():
         extract_rates(name, rel)
     for name, rel in ALL_OPS.items():
         extract_points_only(name, rel)
+        extract_ff_rates(name, rel)
 
 
 if __name__ == "__main__":

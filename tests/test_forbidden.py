@@ -123,3 +123,62 @@ def test_gpu_mixed_strengths_and_crossterms(op, K, mode, grid, monkeypatch):
         want = np.array([complex(*p["crossterms"][lbl]) for p in pts])
         assert np.max(np.abs(got - want)) / np.max(np.abs(want)) < tol_g, lbl
     eng.close()
+
+
+def _engine_rates(mode, eps):
+    """The engine solves the computed half of the archived contour (30 of 60
+    points) for all 14 operator-K sets with convergence threshold eps; the
+    cross-terms come from its amplitudes; returns the 19 beta.out rates and the
+    archived ones."""
+    from pynfam_amd import engine as E
+    from pynfam_amd import ff_rate as FF
+    from pynfam_amd import rate as R
+    from pynfam_amd.crossterms import contract, crossterm_problems
+    from test_rate import load_all_operator_run
+    d, _, _, emax, q = load_all_operator_run()
+    s = d["summary"]
+    ctr = R.circle_contour(0.0, emax, 60)
+    w = ctr["z"][:ctr["n_compute"]]
+    S, G = {}, {}
+    for op, K in OPK:
+        prob = all_problem(op, K)
+        eng = E.GpuEngine(prob)
+        r = eng.solve(w, precision=getattr(E, mode), want_x=True, eps=eps, max_iter=600)
+        assert np.all(r["iterations"] > 0), (op, K, r["iterations"])
+        S[(op, K)] = R.fill_conjugate(r["strength"], 60)
+        for lbl, gp in crossterm_problems(prob):
+            ge = E.GpuEngine(gp)
+            g20, g02 = ge.fqp()
+            ge.close()
+            G[(lbl, K)] = R.fill_conjugate(contract(r["x"], g20, g02), 60)
+        eng.close()
+    rates, _, _ = FF.beta_rates(S, G, s["Zf"], s["A"], emax, q, lam=abs(s["gA"] / s["gV"]))
+    return rates, d["rates"]
+
+
+@pytest.mark.gpu
+def test_gpu_beta_rates_of_all_operators():
+    """End to end in FP64 at the archived run's threshold (si < 1e-7): the 19
+    allowed and first-forbidden rates from the engine's strengths and
+    cross-terms against beta.out, 1e-6 relative (the phase-space continuation
+    alone leaves up to 1.2e-7)."""
+    rates, ref = _engine_rates("FP64", 1e-7)
+    errs = {name: abs(rates[name] - v) / abs(v) for name, v in ref.items()}
+    print(" ".join("%s %.1e" % kv for kv in errs.items()))
+    assert max(errs.values()) < 1e-6, errs
+
+
+@pytest.mark.gpu
+def test_gpu_mixed_beta_rates_against_fp64():
+    """The mixed path's rates against the FP64 path's, both converged to
+    si < 1e-10.  The closed contour cancels strongly on this small basis (the
+    Fermi rate is 4e-3 of the total), so rates compared at si < 1e-7 differ by
+    how far each run converged: FP64 at 1e-7 and at 1e-10 differ by up to 2e-3
+    (Fermi) and 7e-4 (J = 1).  At 1e-10 the mixed path with its FP64
+    refinement leaves 3.2e-5 on the Fermi rate and <= 1.4e-5 elsewhere: 1e-4
+    required."""
+    tight, _ = _engine_rates("FP64", 1e-10)
+    mixed, _ = _engine_rates("MIXED", 1e-10)
+    errs = {name: abs(mixed[name] - v) / abs(v) for name, v in tight.items()}
+    print(" ".join("%s %.1e" % kv for kv in errs.items()))
+    assert max(errs.values()) < 1e-4, errs

@@ -101,3 +101,103 @@ def test_rates_match_archived(nucleus):
     for K in (0, 1):
         assert abs(rates[K] - ref["Allowed-GT_K=%d" % K]) < 1e-8 * ref["Allowed-GT_K=%d" % K], (K, rates[K])
     assert abs(rates["total"] - ref["Total-GT"]) < 1e-8 * ref["Total-GT"]
+
+
+# ---- first-forbidden shape factors and rates (all-operator S40 run) ----------
+
+def load_all_operator_run():
+    """Archived contour tables of the all-operator S40 run as full-contour
+    strengths {(op, K): S} and cross-terms {(label, K): G}, with EQRPA_max (from
+    the contour), the HFB Q value (EQRPA_max - E_gs) and the run's summary."""
+    with open(os.path.join(GOLDEN, "S40_All", "rates.json")) as fh:
+        d = json.load(fh)
+    strengths, cross = {}, {}
+    for name, tab in d["contour"].items():
+        op, k = name.split("-K")
+        cols, a = tab["columns"], np.array(tab["rows"], dtype=float)
+        strengths[(op, int(k))] = a[:, cols.index("Re(Strength)")] + 1j * a[:, cols.index("Im(Strength)")]
+        for c in cols:
+            if c.startswith("Re(") and "x" in c:
+                lbl = c[3:-1]
+                cross[(lbl, int(k))] = a[:, cols.index(c)] + 1j * a[:, cols.index("Im(%s)" % lbl)]
+    a = np.array(d["contour"]["GT-K0"]["rows"], dtype=float)
+    z = a[:, 1] + 1j * a[:, 2]
+    emax = 2.0 * float(np.mean((z / (1.0 + np.exp(1j * a[:, 0]))).real))
+    return d, strengths, cross, emax, emax - d["E_gs"]
+
+
+def _archived_phase(d):
+    pr, pi = np.array(d["phasespace_re"]["rows"], dtype=float), np.array(d["phasespace_im"]["rows"], dtype=float)
+    return {n: pr[:, 3 + n] + 1j * pi[:, 3 + n] for n in range(1, 7)}
+
+
+def test_ff_phase_space_matches_archived():
+    """f1..f6 of the all-operator run; f6 carries lambda_2 = F1/F0 times the
+    Coulomb factor, i.e. the generalised Fermi function F1."""
+    d, _, _, emax, _ = load_all_operator_run()
+    s = d["summary"]
+    ref = _archived_phase(d)
+    pr = np.array(d["phasespace_re"]["rows"], dtype=float)
+    w0 = (emax - (pr[:, 0] + 1j * pr[:, 1])) / R.MEC2 + 1.0
+    for n in range(1, 7):
+        f = R.phase_space_function(n, s["Zf"], s["A"], emax, 0.0)(w0)
+        assert np.max(np.abs(f - ref[n])) < 1e-9 * np.max(np.abs(ref[n])), n
+
+
+def test_generalised_fermi_function_limits():
+    """Z = 0: F_{k-1} = (k (2k-1)!!)^2 4^k Gamma(k)^2 / Gamma(2k+1)^2 = 1 for
+    k = 1, 2 (36 * 16 / 24^2 for k = 2), so lambda_2 = (2 + 2) / (2 * 2) = 1; at
+    and below threshold lambda_2 is zero."""
+    w = np.array([1.2, 3.0, 10.0])
+    assert np.allclose(R.fermi_fk(2, 0.0, 100, w), 1.0, rtol=0, atol=1e-13)
+    assert np.allclose(R.lambda_2(0.0, 100, w), 1.0, rtol=0, atol=1e-13)
+    assert np.all(R.lambda_2(30.0, 100, np.array([0.5, 1.0])) == 0.0)
+
+
+def test_ff_shape_factors_match_archived():
+    """Every beta.out channel's shape factor on the contour, from the archived
+    strengths, cross-terms and phase space: the prefactors, the Coulomb x+/-
+    and each J term pinned by its own channel ((J, K) for J = 0, 1, 2)."""
+    from pynfam_amd import ff_rate as FF
+    d, S, G, emax, q = load_all_operator_run()
+    s = d["summary"]
+    _, sf, _ = FF.beta_rates(S, G, s["Zf"], s["A"], emax, q, lam=abs(s["gA"] / s["gV"]), phase=_archived_phase(d))
+    cols = d["shapefactor_re"]["columns"]
+    ar, ai = np.array(d["shapefactor_re"]["rows"], dtype=float), np.array(d["shapefactor_im"]["rows"], dtype=float)
+    assert set(FF.CHANNELS) == set(cols) - {"Re(EQRPA)", "Im(EQRPA)"}
+    for name in FF.CHANNELS:
+        ref = ar[:, cols.index(name)] + 1j * ai[:, cols.index(name)]
+        assert np.max(np.abs(sf[name] - ref)) < 1e-13 * np.max(np.abs(ref)), name
+
+
+def test_ff_rates_match_archived():
+    """All 19 rates of beta.out: 1e-10 relative with the archived phase space;
+    with this module's RATINT phase space (1e-10 of max |f| off near the
+    endpoint) the closed-contour cancellation leaves 1.2e-7 on the small Fermi
+    rate and <= 2e-8 elsewhere: 1e-6 required."""
+    from pynfam_amd import ff_rate as FF
+    d, S, G, emax, q = load_all_operator_run()
+    s = d["summary"]
+    lam = abs(s["gA"] / s["gV"])
+    ref = d["rates"]
+    assert set(ref) == set(FF.CHANNELS)
+    r1, _, _ = FF.beta_rates(S, G, s["Zf"], s["A"], emax, q, lam=lam, phase=_archived_phase(d))
+    r2, _, _ = FF.beta_rates(S, G, s["Zf"], s["A"], emax, q, lam=lam)
+    for name, v in ref.items():
+        assert abs(r1[name] - v) < 1e-10 * abs(v), (name, r1[name], v)
+        assert abs(r2[name] - v) < 1e-6 * abs(v), (name, r2[name], v)
+    # the channel sums are consistent: per K, per J and per (J, K) all add up to the total
+    tf = r1["Total-Forbidden"]
+    for group in (["Forbidden-K=%d" % k for k in range(3)], ["Forbidden-J=%d" % j for j in range(3)],
+                  [c for c in FF.CHANNELS if c.startswith("Forbidden-(J,K)")]):
+        assert abs(sum(r1[c] for c in group) - tf) < 1e-12 * tf, group
+    assert abs(r1["Total-Allowed"] + tf - r1["Total"]) < 1e-12 * r1["Total"]
+
+
+def test_ff_cross_terms_follow_the_conjugate_fill():
+    """The archived second half of every strength and cross-term column is the
+    conjugate fill of the first half, so the engine computes only the first."""
+    _, S, G, _, _ = load_all_operator_run()
+    for v in list(S.values()) + list(G.values()):
+        n = len(v)
+        assert np.array_equal(R.fill_conjugate(v[:(n + 1) // 2], n), v)
