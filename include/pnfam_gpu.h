@@ -69,7 +69,9 @@ typedef struct {
  *     FP32 accumulation.  The residual, its max-norm, Broyden, G(w) and S stay
  *     FP64.  When a point first
  *     meets the tolerance, one full FP64 evaluation replaces the accumulated H
- *     and the point iterates on until it meets the tolerance again.
+ *     and the point iterates on until it meets the tolerance again; the Broyden
+ *     pair formed across that evaluation is zeroed (environment
+ *     PNFAM_GPU_MIX_DROP=0 keeps it).
  *   MIXED_NOREFINE: as MIXED without the final FP64 evaluation; S then carries
  *     the low-precision error of the first increments (about 1e-6 relative).
  *   Both mixed modes keep the Broyden history in FP32 (environment

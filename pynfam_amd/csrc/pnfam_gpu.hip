@@ -1036,6 +1036,7 @@ struct BroArgs {
   double *vmax;           // [w] max |vin|
   double *scal;           // [w][4]: normi, ...
   double *dvin;           // mixed path: [w][N] the step added to vin (the next increment); null for FP64
+  const int *drop;        // [w] 1: this iteration's history pair is zeroed (mixed refinement); null: none
   long N;
   int M, nwg, iter, ipos, inext, iter_used;
   double alpha;
@@ -1134,7 +1135,10 @@ __global__ void __launch_bounds__(64) k_bro_norm(BroArgs B) {
   if (threadIdx.x != 0) return;
   B.si[w] = m;
   B.vmax[w] = v;
-  B.scal[(size_t)w * 4 + 0] = (B.iter >= 2 && B.M > 0) ? 1.0 / sqrt(s) : 1.0;
+  // a dropped pair is scaled to zero: its Gram row is then w0^2 + 1 on the
+  // diagonal only and its coefficient zero, so it adds nothing to the step
+  const bool drop = B.drop != nullptr && B.drop[w] != 0;
+  B.scal[(size_t)w * 4 + 0] = (B.iter >= 2 && B.M > 0) ? (drop ? 0.0 : 1.0 / sqrt(s)) : 1.0;
 }
 
 // scale the history slot, then partial dots df_j . df_ipos and df_j . vout,
@@ -1389,7 +1393,7 @@ struct pnfam_gpu_ctx {
   DevPtr d_vin, d_vout, d_rsp, d_hsp, d_hqp, d_dens, d_flds, d_part, d_cpart, d_S, d_si, d_vmax, d_scal;
   DevPtr d_df, d_dv, d_pdf, d_gram, d_coef, d_wre, d_wim, d_act;
   // mixed path: the next increment of vin, accumulated FP64 H20/H02
-  DevPtr d_act64, d_actlo, d_dvin, d_hacc;
+  DevPtr d_act64, d_actlo, d_dvin, d_hacc, d_drop;
   // split-BF16 grid stage: one element layout per block structure (the mean-field
   // set in map f, one pairing set per sector), pair tables and operands
   struct MixSet {
@@ -1780,13 +1784,14 @@ int ensure_workspace(pnfam_gpu_ctx *h, int npts, int M) {
     if (h->pin_act) { (void)hipHostFree(h->pin_act); h->pin_act = nullptr; }
     if (h->pin_res) { (void)hipHostFree(h->pin_res); h->pin_res = nullptr; }
     h->pin_cap = 0;
-    if (hipHostMalloc((void **)&h->pin_act, 3 * n * 4) != hipSuccess ||
+    if (hipHostMalloc((void **)&h->pin_act, 4 * n * 4) != hipSuccess ||
         hipHostMalloc((void **)&h->pin_res, 4 * n * 8) != hipSuccess)
       return fail(PNFAM_GPU_ENOMEM, "hipHostMalloc of the pinned staging buffers failed");
     h->pin_cap = (int)n;
   }
   if ((e = h->d_act.alloc(n * 4))) return e;
   if ((e = h->d_act64.alloc(n * 4))) return e;
+  if ((e = h->d_drop.alloc(n * 4))) return e;
   if ((e = h->d_actlo.alloc(n * 4))) return e;
   if ((e = h->d_dvin.alloc(n * 4 * nxy * 8))) return e;
   if ((e = h->d_hacc.alloc(n * 4 * nxy * 8))) return e;
@@ -3014,10 +3019,18 @@ int pnfam_gpu_solve_batch(pnfam_gpu_ctx *h, int32_t npts, const double *re_w, co
   // PNFAM_GPU_TRACE=1: one line per iteration on stderr (active points, BF16
   // products of the grid stage, wall time of the iteration)
   const bool trace = std::getenv("PNFAM_GPU_TRACE") != nullptr;
+  // The refinement's FP64 evaluation moves the residual by the error the
+  // accumulated field carried.  That jump is not a secant of the fixed-point map,
+  // so the Broyden pair formed across it is zeroed (PNFAM_GPU_MIX_DROP=0 keeps it).
+  bool drop_pairs = mixed && refine && !skip_res && M > 0;
+  if (const char *ev = std::getenv("PNFAM_GPU_MIX_DROP")) drop_pairs = drop_pairs && ev[0] != '0';
+  int *pin_drop = h->pin_act + 3 * npts;
   auto t_last = std::chrono::steady_clock::now();
   int n3_last = 0;
   for (int iter = 0; iter <= o->max_iter; ++iter) {
     act.clear(); act64.clear(); actlo.clear();
+    bool any_drop = false;
+    if (drop_pairs) std::fill(pin_drop, pin_drop + npts, 0);
     for (int w = 0; w < npts; ++w) {
       if (done[w]) continue;
       if (si[w] < o->convergence_epsilon) {
@@ -3026,6 +3039,7 @@ int pnfam_gpu_solve_batch(pnfam_gpu_ctx *h, int32_t npts, const double *re_w, co
           // it, and the point iterates on until it meets the tolerance again
           refined[w] = 1;
           act.push_back(w); act64.push_back(w);
+          if (drop_pairs) { pin_drop[w] = 1; any_drop = true; }
           continue;
         }
         done[w] = 1; iters[w] = iter; continue;
@@ -3095,6 +3109,10 @@ int pnfam_gpu_solve_batch(pnfam_gpu_ctx *h, int32_t npts, const double *re_w, co
     B.coef = h->d_coef.as<double>(); B.si = h->d_si.as<double>(); B.scal = h->d_scal.as<double>();
     B.vmax = h->d_vmax.as<double>();
     B.dvin = mixed ? h->d_dvin.as<double>() : nullptr;
+    if (any_drop) {
+      HIP_TRY(hipMemcpyAsync(h->d_drop.p, pin_drop, npts * 4, hipMemcpyHostToDevice, st));
+      B.drop = h->d_drop.as<int>();
+    }
     // with the FP32 history (its reductions need not match the FP64 path's order)
     // and few points, more workgroups per point keep more loads in flight
     const int nwg_it = (hist32 && na <= BRO_FEW) ? h->bro_nwg_few : nwg;

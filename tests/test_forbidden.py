@@ -175,7 +175,7 @@ def test_gpu_mixed_beta_rates_against_fp64():
     Fermi rate is 4e-3 of the total), so rates compared at si < 1e-7 differ by
     how far each run converged: FP64 at 1e-7 and at 1e-10 differ by up to 2e-3
     (Fermi) and 7e-4 (J = 1).  At 1e-10 the mixed path with its FP64
-    refinement leaves 3.2e-5 on the Fermi rate and <= 1.4e-5 elsewhere: 1e-4
+    refinement leaves 3.3e-5 on the Fermi rate and <= 1.5e-5 elsewhere: 1e-4
     required."""
     tight, _ = _engine_rates("FP64", 1e-10)
     mixed, _ = _engine_rates("MIXED", 1e-10)
