@@ -219,3 +219,32 @@ def test_calc_hamiltonian_plain_kernels(monkeypatch, K):
     for k in range(4):
         assert rel(got_q[k], ref_q[k]) < 1e-13
     eng.close()
+
+
+@pytest.mark.parametrize("nucleus", ["S40", "Gd162"])
+def test_gt_rates_end_to_end(nucleus):
+    """The engine solves the computed half of each archived GT contour (K = 0, 1)
+    in FP64; the allowed GT rates follow from its strengths alone and are
+    compared with beta.out: 1e-6 relative (the phase-space continuation leaves
+    1e-8 with the archived strengths)."""
+    from pynfam_amd import rate as R
+    from pynfam_amd.engine import GpuEngine
+    from test_rate import _eqrpa_max, _load
+    d, arr = _load(nucleus)
+    s = d["summary"]
+    emax = _eqrpa_max(d, arr)
+    npts = len(arr(d["GT-K0"]))
+    ctr = R.circle_contour(0.0, emax, npts)
+    half = {}
+    for K in (0, 1):
+        eng = GpuEngine(golden_problem(nucleus, K))
+        r = eng.solve(ctr["z"][:ctr["n_compute"]])
+        eng.close()
+        assert np.all(r["iterations"] > 0), r["iterations"]
+        half[K] = r["strength"]
+    rates, _, _ = R.gt_rates_from_half(s["Zf"], s["A"], emax, 0.0, npts, half)
+    ref = d["rates"]
+    errs = {K: abs(rates[K] - ref["Allowed-GT_K=%d" % K]) / ref["Allowed-GT_K=%d" % K] for K in (0, 1)}
+    errs["total"] = abs(rates["total"] - ref["Total-GT"]) / ref["Total-GT"]
+    print(nucleus, errs)
+    assert max(errs.values()) < 1e-6, errs
