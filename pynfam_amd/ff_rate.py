@@ -56,6 +56,8 @@ def strength_products(strengths, crossterms, npts):
     computed contribute zero, as in the reference."""
     b = {key: {K: np.zeros(npts, dtype=complex) for K in range(3)} for key in OPS + tuple(c[0] for c in CROSS)}
     for (op, K), s in strengths.items():
+        if op not in OPS or not 0 <= K <= 2:
+            raise ValueError(f"no shape-factor term for operator {op} with K={K}")
         b[op][K] = prefactor(op, K) ** 2 * np.asarray(s, dtype=complex)
     for key, op, label, partner in CROSS:
         for K in range(3):

@@ -201,3 +201,11 @@ def test_ff_cross_terms_follow_the_conjugate_fill():
     for v in list(S.values()) + list(G.values()):
         n = len(v)
         assert np.array_equal(R.fill_conjugate(v[:(n + 1) // 2], n), v)
+
+
+def test_ff_rejects_unknown_operators():
+    """Operators without a shape-factor term (the I-function set, two-body
+    currents) are refused rather than dropped."""
+    from pynfam_amd import ff_rate as FF
+    with pytest.raises(ValueError):
+        FF.strength_products({("RS0I", 0): np.ones(4)}, {}, 4)
