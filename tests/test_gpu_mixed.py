@@ -125,3 +125,27 @@ def test_mixed_edge_cases():
         assert r["iterations"][0] > 0
         assert abs(r["strength"][0] - ref["strength"][0]) < 1e-4 * abs(ref["strength"][0])
     eng.close()
+
+
+@pytest.mark.parametrize("mode", ["MIXED", "MIXED_NOREFINE"])
+def test_mixed_beta_plus(mode):
+    """beta+ swaps the proton and neutron roles in the transforms (Fqp = W_n^T F W_p)
+    and in G(w); the mixed path against the FP64 path on the S40 beta+ GT K=1
+    problem, with the tolerances of the archived runs."""
+    import os
+    from conftest import GOLDEN
+    from pynfam_amd import engine as E
+    from pynfam_amd.problem import build_problem
+    nml = golden_points("S40", 1)[0]["namelist"]
+    prob = build_problem(os.path.join(GOLDEN, "S40", "hfbtho_output.hel"), "GT", 1, "+", "SKOP",
+                         vpair_t0=nml["vpair_t0"], override_cs0=nml["override_cs0"], override_csr=0.0,
+                         override_cds=0.0)
+    eng = E.GpuEngine(prob)
+    w = np.array([2.0 + 0.5j, 4.0 + 0.5j, 9.0 + 0.5j, 15.0 + 2.0j])
+    ref = eng.solve(w)
+    r = eng.solve(w, precision=getattr(E, mode))
+    eng.close()
+    assert np.all(r["iterations"] > 0), r["iterations"]
+    err = np.max(np.abs(r["strength"] - ref["strength"]) / np.abs(ref["strength"]))
+    print(f"beta+ {mode}: max rel err {err:.2e}, iterations {r['iterations'].tolist()} vs {ref['iterations'].tolist()}")
+    assert err < TOL[mode], err
