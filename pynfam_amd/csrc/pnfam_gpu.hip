@@ -260,6 +260,7 @@ struct DensArgs {
   PairList rho[2], kap[2];   // per sector
   int rho_slot[2], kap_slot[2];
   double *dens;        // [w][2][56][nghl]
+  double *part;        // k_density_mfma split over block pairs: [split][active index][2][56][nghl]; null: dens
   const int *act;
 };
 
@@ -479,7 +480,11 @@ __device__ inline void density_column(double *acc, const double (&Tr)[4][2], con
 }
 
 // Local densities with FP64 MFMA (v_mfma_f64_16x16x4).  One wave per
-// (16 grid points, frequency, sector), looping over all block pairs.
+// (16 grid points, frequency, sector), looping over all block pairs.  With few
+// frequencies the third grid dimension also splits the pair lists
+// (blockIdx.z = sector + 2 split): split z takes pairs [z n / Z, (z + 1) n / Z)
+// of each list and writes its partial densities to A.part, which k_dens_sum
+// adds split by split.  With one split the kernel is the unsplit one.
 // T_x^s(r, b) = sum_a psi_x(r,a) rho(a,b) runs on the matrix cores (rows r,
 // k = a in steps of 4, 16 columns b per tile, re and im tiles), per-spin ranges
 // of a accumulated separately; the tile goes through LDS to a lane-per-(r, 4 b)
@@ -489,7 +494,8 @@ __global__ void __launch_bounds__(64) k_density_mfma(DensArgs A) {
   const int lane = threadIdx.x;
   const int r0 = blockIdx.x * 16;
   const int w = A.act[blockIdx.y];
-  const int sec = blockIdx.z;
+  const int sec = blockIdx.z & 1;
+  const int spl = blockIdx.z >> 1, nspl = gridDim.z >> 1;
   const int G = A.nghl;
   const int mi = lane & 15, mk = lane >> 4;     // MFMA operand lane coordinates
   const int er = lane & 15, eg = lane >> 4;     // epilogue: row er, columns eg + 4q
@@ -505,7 +511,8 @@ __global__ void __launch_bounds__(64) k_density_mfma(DensArgs A) {
   const double *rho_im = rho_re + A.nxy;
   const double *tab[4] = {A.wf, A.wfdr, A.wfdp, A.wfdz};
   const PairList pl = A.rho[sec];
-  for (int p = 0; p < pl.n; ++p) {
+  const int p0 = (int)((long)spl * pl.n / nspl), p1 = (int)((long)(spl + 1) * pl.n / nspl);
+  for (int p = p0; p < p1; ++p) {
     const int4 pr = pl.p[p];
     const int ix = pr.x, iy = pr.y, off = pr.z;
     const int ia = A.isstart[ix], ib0 = A.isstart[iy], dx = A.db[ix], dy = A.db[iy], nsu = A.nsu[ix];
@@ -568,7 +575,8 @@ __global__ void __launch_bounds__(64) k_density_mfma(DensArgs A) {
   const double *kap_re = A.rsp + ((size_t)w * 8 + 2 * A.kap_slot[sec]) * A.nxy;
   const double *kap_im = kap_re + A.nxy;
   const PairList kl = A.kap[sec];
-  for (int p = 0; p < kl.n; ++p) {
+  const int q0 = (int)((long)spl * kl.n / nspl), q1 = (int)((long)(spl + 1) * kl.n / nspl);
+  for (int p = q0; p < q1; ++p) {
     const int4 pr = kl.p[p];
     const int ix = pr.x, iy = pr.y, off = pr.z;
     const int ia = A.isstart[ix], ib0 = A.isstart[iy], dx = A.db[ix], dy = A.db[iy], nsu = A.nsu[ix];
@@ -617,13 +625,27 @@ __global__ void __launch_bounds__(64) k_density_mfma(DensArgs A) {
   }
   // sum the four column subgroups (lanes er, er+16, er+32, er+48)
   const double wc = rE_ok ? A.wdcori[rE] : 0.0;
-  double *out = A.dens + (((size_t)w * 2 + sec) * 2 * NDENS) * G + rEc;
+  double *out = (A.part ? A.part + (((size_t)spl * gridDim.y + blockIdx.y) * 2 + sec) * 2 * NDENS * G
+                        : A.dens + (((size_t)w * 2 + sec) * 2 * NDENS) * G) + rEc;
 #pragma unroll
   for (int d = 0; d < 2 * NDENS; ++d) {
     double v = acc[d];
     v += __shfl_xor(v, 16, 64);
     v += __shfl_xor(v, 32, 64);
     if (eg == 0 && rE_ok) out[(size_t)d * G] = v * wc;
+  }
+}
+
+// densities of the active frequency y = the sum of its nsplit partial
+// densities, split by split (k_density_mfma with split pair lists)
+__global__ void k_dens_sum(const double *__restrict__ part, int nsplit, int na, long nd, const int *__restrict__ act,
+                           double *__restrict__ dens) {
+  const int y = blockIdx.y;
+  double *out = dens + (size_t)act[y] * nd;
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < nd; e += (long)gridDim.x * blockDim.x) {
+    double v = 0.0;
+    for (int s = 0; s < nsplit; ++s) v += part[((size_t)s * na + y) * nd + e];
+    out[e] = v;
   }
 }
 
@@ -752,6 +774,7 @@ struct ProjArgs {
   const int *isstart, *db, *nsu, *ns;
   const double *flds;     // [w][2][208][nghl]
   double *hsp;            // [w][8][nxy]
+  double *part;           // k_project_mfma split over grid points: [split][active index][8][nxy]; null: hsp
   long nxy;
   int nghl;
   const Tile *tiles;
@@ -861,6 +884,11 @@ struct Tile2 {        // one wave's output of a projected matrix: rows of one sp
 static_assert(sizeof(Tile2) == 48, "Tile2 layout");
 
 constexpr int P2_MAXRT = 4;   // row tiles of 16 per wave
+// FP64 density and projection with few frequencies: splits of the serial loop
+// (pairs, grid points) per frequency, and the most (split, frequency) partial
+// outputs they write
+constexpr int FEW_SPLIT_SLABS = 32;
+inline int few_split(int na) { return na <= 4 ? 8 : (na <= 8 ? 4 : 1); }
 
 // h_ab / Delta_ab with FP64 MFMA: D[a][b] = sum_{z,r} psi_z(r,a) B_z(r,b), with
 // B_z(r,b) = sum_z' mf(z,z',s_a,s_b)(r) psi_z'(r,b) built in registers from
@@ -872,6 +900,10 @@ constexpr int P2_MAXRT = 4;   // row tiles of 16 per wave
 // Grid (tile, frequency), tiles fastest.  Ordering frequencies fastest within
 // an XCD (so a tile's psi rows are fetched once per L2) cut the HBM fetch 4x
 // but ran 1.4x slower: the kernel is not bound by HBM traffic.
+// With few frequencies a third grid dimension splits the grid points: split z
+// takes the 16-point chunks [z n / Z, (z + 1) n / Z) and writes its partial
+// tile to P.part, and k_proj_sum adds the splits in order.  With one split the
+// loop and the output are those of the unsplit kernel.
 template <int NRT, int KIND>
 __global__ void __launch_bounds__(64) k_project_mfma(ProjArgs P, const Tile2 *__restrict__ tiles) {
   constexpr int NZ = KIND == 0 ? 5 : 1;
@@ -891,7 +923,9 @@ __global__ void __launch_bounds__(64) k_project_mfma(ProjArgs P, const Tile2 *__
   v4d C[NRT][2];
 #pragma unroll
   for (int q = 0; q < NRT; ++q) { C[q][0] = v4d{0, 0, 0, 0}; C[q][1] = v4d{0, 0, 0, 0}; }
-  for (int rc = 0; rc < G; rc += 16) {
+  const int nch = (G + 15) / 16;
+  const int c0 = (int)((long)blockIdx.z * nch / gridDim.z), c1 = (int)((long)(blockIdx.z + 1) * nch / gridDim.z);
+  for (int rc = 16 * c0; rc < 16 * c1; rc += 16) {
     // stage the fields of 16 grid points for this spin of a; lanes run along r
     if (KIND == 0) {
       for (int e = lane; e < 16 * 2 * 25 * 2; e += 64) {
@@ -948,7 +982,9 @@ __global__ void __launch_bounds__(64) k_project_mfma(ProjArgs P, const Tile2 *__
     }
     __syncthreads();
   }
-  double *ore = P.hsp + ((size_t)w * 8 + 2 * t.out_slot) * P.nxy + t.off;
+  double *obase = P.part ? P.part + ((size_t)blockIdx.z * gridDim.y + blockIdx.y) * 8 * P.nxy
+                         : P.hsp + (size_t)w * 8 * P.nxy;
+  double *ore = obase + (size_t)(2 * t.out_slot) * P.nxy + t.off;
   double *oim = ore + P.nxy;
   const int bcolo = t.b0 + (lane & 15);
 #pragma unroll
@@ -962,6 +998,19 @@ __global__ void __launch_bounds__(64) k_project_mfma(ProjArgs P, const Tile2 *__
         oim[o] = 2.0 * C[q][1][v];
       }
     }
+  }
+}
+
+// hsp of the active frequency y = the sum of its nsplit partial projections,
+// split by split (k_project_mfma with a split grid)
+__global__ void k_proj_sum(const double *__restrict__ part, int nsplit, int na, long n8, const int *__restrict__ act,
+                           double *__restrict__ hsp) {
+  const int y = blockIdx.y;
+  double *out = hsp + (size_t)act[y] * n8;
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n8; e += (long)gridDim.x * blockDim.x) {
+    double v = 0.0;
+    for (int s = 0; s < nsplit; ++s) v += part[((size_t)s * na + y) * n8 + e];
+    out[e] = v;
   }
 }
 
@@ -1394,6 +1443,8 @@ struct pnfam_gpu_ctx {
   DevPtr d_df, d_dv, d_pdf, d_gram, d_coef, d_wre, d_wim, d_act;
   // mixed path: the next increment of vin, accumulated FP64 H20/H02
   DevPtr d_act64, d_actlo, d_dvin, d_hacc, d_drop;
+  DevPtr d_ppart;   // partial projections of a grid-point split (few frequencies)
+  DevPtr d_dpart;   // partial densities of a pair split (few frequencies)
   // split-BF16 grid stage: one element layout per block structure (the mean-field
   // set in map f, one pairing set per sector), pair tables and operands
   struct MixSet {
@@ -1863,8 +1914,19 @@ int stage_hamiltonian(pnfam_gpu_ctx *h, int na, const int *act) {
   { Prof pr(h, K_DENSITY, na);
     if (h->basic_kernels)
       hipLaunchKernelGGL(k_density, dim3((h->nghl + 63) / 64, na, 2), dim3(64), 0, h->stream, A);
-    else
-      hipLaunchKernelGGL(k_density_mfma, dim3((h->nghl + 15) / 16, na, 2), dim3(64), 0, h->stream, A); }
+    else {
+      // few frequencies: split the pair lists so that the waves fill the device
+      const int nsplit = few_split(na);
+      const long nd = 2L * 2 * NDENS * h->nghl;
+      if (nsplit > 1) {
+        if (int e = h->d_dpart.alloc((size_t)FEW_SPLIT_SLABS * nd * 8)) return e;
+        A.part = h->d_dpart.as<double>();
+      }
+      hipLaunchKernelGGL(k_density_mfma, dim3((h->nghl + 15) / 16, na, 2 * nsplit), dim3(64), 0, h->stream, A);
+      if (nsplit > 1)
+        hipLaunchKernelGGL(k_dens_sum, dim3((unsigned)std::min<long>(1024, (nd + 255) / 256), na), dim3(256), 0,
+                           h->stream, A.part, nsplit, na, nd, act, h->d_dens.as<double>());
+    } }
   if (int e = stage_fields(h, na, act)) return e;
   { Prof pr(h, K_ZERO, na);
     hipLaunchKernelGGL(k_zero_slots, dim3(64, na), dim3(256), 0, h->stream, h->d_hsp.as<double>(), act, 8, 0, 8, h->nxy); }
@@ -1880,10 +1942,21 @@ int stage_hamiltonian(pnfam_gpu_ctx *h, int na, const int *act) {
   } else if (h->ntiles2) {
     Prof pr(h, K_PROJECT, na);
     const Tile2 *tl = h->d_tiles2.as<Tile2>();
+    // few frequencies: split the grid points so that the waves fill the device
+    // (one frequency leaves about one wave per SIMD with a serial loop over all
+    // grid points)
+    const int nsplit = few_split(na);
+    const long n8 = 8 * h->nxy;
+    if (nsplit > 1) {
+      if (int e = h->d_ppart.alloc((size_t)FEW_SPLIT_SLABS * n8 * 8)) return e;
+      P.part = h->d_ppart.as<double>();
+      hipLaunchKernelGGL(k_fill, grid1((size_t)nsplit * na * n8, 256), dim3(256), 0, h->stream, P.part,
+                         (size_t)nsplit * na * n8, 0.0);
+    }
     for (int k = 0; k < 16; ++k) {
       const int n = h->pj_count[k];
       if (!n) continue;
-      const dim3 grid(n, na);
+      const dim3 grid(n, na, nsplit);
       const Tile2 *tk = tl + h->pj_first[k];
 #define PJ_LAUNCH(NRT, KIND) hipLaunchKernelGGL((k_project_mfma<NRT, KIND>), grid, dim3(64), 0, h->stream, P, tk)
       switch (k) {
@@ -1899,6 +1972,9 @@ int stage_hamiltonian(pnfam_gpu_ctx *h, int na, const int *act) {
       }
 #undef PJ_LAUNCH
     }
+    if (nsplit > 1)
+      hipLaunchKernelGGL(k_proj_sum, dim3((unsigned)std::min<long>(1024, (n8 + 255) / 256), na), dim3(256), 0,
+                         h->stream, P.part, nsplit, na, n8, act, h->d_hsp.as<double>());
   }
   HIP_TRY(hipGetLastError());
   return 0;
